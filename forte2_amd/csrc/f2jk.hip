@@ -1,0 +1,605 @@
+// C ABI of the MI355X density-fitted J/K builder.  Declarations and the
+// reference functions each entry point replaces are in include/f2jk.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "f2jk.h"
+#include "f2jk_kernels.h"
+
+using namespace f2jk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define F2JK_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(F2JK_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define F2JK_LAUNCHED(what)                                                             \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess)                                                           \
+            return fail(F2JK_EHIP, std::string("launch ") + what + ": " +               \
+                                       hipGetErrorString(e_));                         \
+    } while (0)
+
+constexpr int kMaxDens = 8;
+constexpr int kMaxEvents = 512;
+constexpr int kEPT = 4;                 // vector elements per thread in the J kernels
+constexpr int kJ1PerBlock = 32;         // P values per block in J pass 1
+constexpr int kTargetBlocks = 2048;     // blocks wanted per launch to fill 256 CUs
+constexpr size_t kDefaultXChunkBytes = size_t(4) << 30;
+
+size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct f2jk_ctx {
+    int device = 0;
+    const void* B[2] = {nullptr, nullptr};
+    int64_t naux = 0, nbf = 0;
+    int* rowoff[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // by VEC (1, 2, 4)
+    int nvec[5] = {0, 0, 0, 0, 0};
+    size_t xchunk_bytes = kDefaultXChunkBytes;
+    int target_blocks = kTargetBlocks;
+    // timing
+    hipEvent_t ev[kMaxEvents];      // (start, end) pairs of non-nested phases
+    int ev_phase[kMaxEvents / 2];
+    int nev = 0;
+    hipEvent_t ev_total[2];         // the whole build
+    bool total_recorded = false;
+    bool timing = false;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Launch plan and workspace layout of one build.
+// ---------------------------------------------------------------------------
+struct Plan {
+    int N = 0, Q = 0, nd = 0;
+    bool want_j = false, want_k = false, need_c = false;
+    int prec = 0;
+    size_t tk = 8;             // bytes of a K-path element (8 fp64, 4 fp32)
+    // C packing (K and the D of build_jk)
+    int OPAD = 0;
+    int width[kMaxDens] = {}, ioff[kMaxDens] = {};
+    // J
+    int jvec = 1, nvec = 0, neg = 0, npc1 = 0, ppw2 = 0, npc2 = 0;
+    size_t jb = 8;             // bytes of a B element read by J
+    // K
+    int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1;
+    // workspace offsets
+    size_t off_cp = 0, off_d = 0, off_dpart = 0, off_gpart = 0, off_gamma = 0, off_jpart = 0,
+           off_x = 0, off_kpart = 0, total = 0;
+};
+
+int j_vec_for(int dtype, int64_t N) {
+    if (dtype == F2JK_F64) return (N % 2 == 0) ? 2 : 1;
+    return (N % 4 == 0) ? 4 : ((N % 2 == 0) ? 2 : 1);
+}
+
+int ensure_rowoff(f2jk_ctx* ctx, int vec) {
+    if (ctx->rowoff[vec]) return F2JK_OK;
+    const int64_t N = ctx->nbf;
+    std::vector<int> h(N + 1);
+    int64_t acc = 0;
+    for (int64_t m = 0; m < N; ++m) {
+        h[m] = (int)acc;
+        acc += (m + 1 + vec - 1) / vec;
+    }
+    h[N] = (int)acc;
+    if (acc > (int64_t)INT32_MAX) return fail(F2JK_EUNSUPPORTED, "nbf too large for J element tables");
+    F2JK_HIP(hipSetDevice(ctx->device));
+    F2JK_HIP(hipMalloc(&ctx->rowoff[vec], sizeof(int) * (N + 1)));
+    F2JK_HIP(hipMemcpy(ctx->rowoff[vec], h.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    ctx->nvec[vec] = (int)acc;
+    return F2JK_OK;
+}
+
+int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
+              bool j_from_c, Plan& p) {
+    if (ctx->nbf <= 0 || ctx->naux <= 0) return fail(F2JK_ESTATE, "B has not been set");
+    if (ndens < 1 || ndens > kMaxDens)
+        return fail(F2JK_EINVAL, "ndens must be in [1, " + std::to_string(kMaxDens) + "]");
+    if (precision < F2JK_PREC_FP64 || precision > F2JK_PREC_BF16X3)
+        return fail(F2JK_EINVAL, "unknown precision");
+    if (precision == F2JK_PREC_BF16X3)
+        return fail(F2JK_EUNSUPPORTED, "BF16X3 precision is not available in this build");
+    p.N = (int)ctx->nbf;
+    p.Q = (int)ctx->naux;
+    p.nd = ndens;
+    p.prec = precision;
+    p.want_j = flags & F2JK_WANT_J;
+    p.want_k = flags & F2JK_WANT_K;
+    p.need_c = p.want_k || (p.want_j && j_from_c);
+    const int jdtype = (precision == F2JK_PREC_FP64) ? F2JK_F64 : F2JK_F32;
+    const int kdtype = jdtype;
+    if (p.want_j && !ctx->B[jdtype])
+        return fail(F2JK_ESTATE, jdtype == F2JK_F64 ? "FP64 B not set" : "FP32 B not set");
+    if (p.want_k && !ctx->B[kdtype])
+        return fail(F2JK_ESTATE, kdtype == F2JK_F64 ? "FP64 B not set" : "FP32 B not set");
+    p.tk = (kdtype == F2JK_F64) ? 8 : 4;
+    const int N = p.N, Q = p.Q;
+
+    size_t off = 0;
+    if (p.need_c) {
+        if (!nocc) return fail(F2JK_EINVAL, "nocc is required");
+        int o = 0;
+        for (int d = 0; d < ndens; ++d) {
+            if (nocc[d] < 0 || nocc[d] > N) return fail(F2JK_EINVAL, "nocc out of range");
+            p.width[d] = (int)(cdiv(std::max<int64_t>(nocc[d], 1), 16) * 16);
+            p.ioff[d] = o;
+            o += p.width[d];
+        }
+        p.OPAD = o;
+        // the C copy used for D (always fp64) and the K copy (fp64 or fp32)
+        p.off_cp = off;
+        off += align_up((size_t)N * p.OPAD * 8);
+        off += align_up((size_t)N * p.OPAD * 4);  // fp32 copy (only used by FP32 K)
+    }
+    if (p.want_j) {
+        p.jb = (jdtype == F2JK_F64) ? 8 : 4;
+        p.jvec = j_vec_for(jdtype, N);
+        int64_t nvec = 0;
+        for (int64_t m = 0; m < N; ++m) nvec += (m + 1 + p.jvec - 1) / p.jvec;
+        p.nvec = (int)nvec;
+        p.neg = (int)cdiv(nvec, (int64_t)kJThreads * kEPT);
+        p.npc1 = (int)cdiv(Q, kJ1PerBlock);
+        p.npc2 = (int)std::max<int64_t>(1, std::min<int64_t>(Q, cdiv(ctx->target_blocks, p.neg)));
+        p.ppw2 = (int)cdiv(Q, p.npc2);
+        p.npc2 = (int)cdiv(Q, p.ppw2);
+        if (j_from_c) {
+            p.off_d = off;
+            off += align_up((size_t)ndens * N * N * 8);
+            p.off_dpart = off;
+            off += align_up((size_t)N * N * 8);
+        }
+        p.off_gpart = off;
+        off += align_up((size_t)p.neg * ndens * Q * 8);
+        p.off_gamma = off;
+        off += align_up((size_t)ndens * Q * 8);
+        p.off_jpart = off;
+        off += align_up((size_t)p.npc2 * ndens * (size_t)nvec * p.jvec * 8);
+    }
+    if (p.want_k) {
+        const size_t per_p = (size_t)N * p.OPAD * p.tk;
+        int64_t pc = (int64_t)(ctx->xchunk_bytes / std::max<size_t>(per_p, 1));
+        pc = std::max<int64_t>(1, std::min<int64_t>(pc, Q));
+        p.Pc = (int)pc;
+        p.nchunks = (int)cdiv(Q, pc);
+        p.ntile = (int)cdiv(N, kTile);
+        p.nlower = p.ntile * (p.ntile + 1) / 2;
+        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(ctx->target_blocks / 2, p.nlower)));
+        p.ncolt = (int)cdiv(p.OPAD, kTile);
+        if (p.tk == 8)
+            p.kva = (N % 2 == 0) ? 2 : 1;
+        else
+            p.kva = (N % 4 == 0) ? 4 : ((N % 2 == 0) ? 2 : 1);
+        p.off_x = off;
+        off += align_up((size_t)p.Pc * per_p);
+        p.off_kpart = off;
+        off += align_up((size_t)p.nsplit * ndens * N * N * 8);
+    }
+    p.total = std::max<size_t>(off, 256);
+    return F2JK_OK;
+}
+
+// --------------------------------------------------------------------------- timing
+void tim_begin(f2jk_ctx* ctx, bool on) {
+    ctx->timing = on;
+    ctx->nev = 0;
+    ctx->total_recorded = false;
+}
+int tim_mark(f2jk_ctx* ctx, int phase, hipStream_t st, bool start) {
+    if (!ctx->timing) return F2JK_OK;
+    if (phase == F2JK_PH_TOTAL) {
+        F2JK_HIP(hipEventRecord(ctx->ev_total[start ? 0 : 1], st));
+        if (!start) ctx->total_recorded = true;
+        return F2JK_OK;
+    }
+    if (start && ctx->nev + 2 > kMaxEvents) return F2JK_OK;  // table full: phase not timed
+    if (!start && (ctx->nev % 2) == 0) return F2JK_OK;       // end of an untimed phase
+    if (start) ctx->ev_phase[ctx->nev / 2] = phase;
+    F2JK_HIP(hipEventRecord(ctx->ev[ctx->nev], st));
+    ctx->nev++;
+    return F2JK_OK;
+}
+#define TIM(phase, start)                                        \
+    do {                                                         \
+        int rc_ = tim_mark(ctx, phase, st, start);               \
+        if (rc_) return rc_;                                     \
+    } while (0)
+
+// --------------------------------------------------------------------------- J
+template <typename TB, int VEC, int ND>
+int launch_j(f2jk_ctx* ctx, const Plan& p, const TB* B, const double* const* D, int64_t ldd,
+             double* const* J, char* ws, hipStream_t st) {
+    DPtrs<ND> dp;
+    JPtrs<ND> jp;
+    for (int d = 0; d < ND; ++d) {
+        dp.p[d] = D[d];
+        jp.p[d] = J[d];
+    }
+    double* gpart = (double*)(ws + p.off_gpart);
+    double* gamma = (double*)(ws + p.off_gamma);
+    double* jpart = (double*)(ws + p.off_jpart);
+    const int* rowoff = ctx->rowoff[VEC];
+    TIM(F2JK_PH_JGAMMA, true);
+    hipLaunchKernelGGL((j_gamma_kernel<TB, VEC, kEPT, ND>), dim3(p.neg, p.npc1), dim3(kJThreads), 0,
+                       st, B, p.N, p.Q, dp, ldd, rowoff, p.nvec, kJ1PerBlock, gpart);
+    F2JK_LAUNCHED("j_gamma");
+    const int64_t ng = (int64_t)ND * p.Q;
+    hipLaunchKernelGGL(j_gamma_reduce_kernel, dim3((unsigned)cdiv(ng, 256)), dim3(256), 0, st,
+                       gpart, p.neg, ND, p.Q, gamma);
+    F2JK_LAUNCHED("j_gamma_reduce");
+    TIM(F2JK_PH_JGAMMA, false);
+    TIM(F2JK_PH_JACC, true);
+    hipLaunchKernelGGL((j_accum_kernel<TB, VEC, kEPT, ND>), dim3(p.neg, p.npc2), dim3(kJThreads), 0,
+                       st, B, p.N, p.Q, gamma, rowoff, p.nvec, p.ppw2, jpart);
+    F2JK_LAUNCHED("j_accum");
+    hipLaunchKernelGGL((j_reduce_kernel<VEC, ND>), dim3((unsigned)cdiv(p.nvec, 256)), dim3(256), 0,
+                       st, jpart, p.npc2, p.N, rowoff, p.nvec, jp);
+    F2JK_LAUNCHED("j_reduce");
+    TIM(F2JK_PH_JACC, false);
+    return F2JK_OK;
+}
+
+template <typename TB, int VEC>
+int launch_j_nd(f2jk_ctx* ctx, const Plan& p, const TB* B, int nd, const double* const* D,
+                int64_t ldd, double* const* J, char* ws, hipStream_t st) {
+    // densities are processed two at a time: one read of B serves both
+    int d = 0;
+    while (d < nd) {
+        int rc;
+        if (nd - d >= 2)
+            rc = launch_j<TB, VEC, 2>(ctx, p, B, D + d, ldd, J + d, ws, st), d += 2;
+        else
+            rc = launch_j<TB, VEC, 1>(ctx, p, B, D + d, ldd, J + d, ws, st), d += 1;
+        if (rc) return rc;
+    }
+    return F2JK_OK;
+}
+
+int run_j(f2jk_ctx* ctx, const Plan& p, const double* const* D, int64_t ldd, double* const* J,
+          char* ws, hipStream_t st) {
+    int rc = ensure_rowoff(ctx, p.jvec);
+    if (rc) return rc;
+    if (p.prec == F2JK_PREC_FP64) {
+        const double* B = (const double*)ctx->B[F2JK_F64];
+        if (p.jvec == 2) return launch_j_nd<double, 2>(ctx, p, B, p.nd, D, ldd, J, ws, st);
+        return launch_j_nd<double, 1>(ctx, p, B, p.nd, D, ldd, J, ws, st);
+    }
+    const float* B = (const float*)ctx->B[F2JK_F32];
+    if (p.jvec == 4) return launch_j_nd<float, 4>(ctx, p, B, p.nd, D, ldd, J, ws, st);
+    if (p.jvec == 2) return launch_j_nd<float, 2>(ctx, p, B, p.nd, D, ldd, J, ws, st);
+    return launch_j_nd<float, 1>(ctx, p, B, p.nd, D, ldd, J, ws, st);
+}
+
+// --------------------------------------------------------------------------- C packing, D
+int pack_c(f2jk_ctx* ctx, const Plan& p, const double* const* C, const int64_t* nocc,
+           const int64_t* ldc, char* ws, hipStream_t st, bool want_f32) {
+    double* cp = (double*)(ws + p.off_cp);
+    float* cp32 = (float*)(ws + p.off_cp + align_up((size_t)p.N * p.OPAD * 8));
+    for (int d = 0; d < p.nd; ++d) {
+        if (nocc[d] > 0 && (!C[d] || !ldc || ldc[d] < nocc[d]))
+            return fail(F2JK_EINVAL, "C pointer / ldc invalid");
+        const int64_t n = (int64_t)p.N * p.width[d];
+        hipLaunchKernelGGL(pad_c_kernel<double>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st,
+                           C[d], nocc[d] > 0 ? ldc[d] : 1, p.N, (int)nocc[d], p.width[d], p.ioff[d],
+                           p.OPAD, cp);
+        F2JK_LAUNCHED("pad_c");
+        if (want_f32) {
+            hipLaunchKernelGGL(pad_c_kernel<float>, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st,
+                               C[d], nocc[d] > 0 ? ldc[d] : 1, p.N, (int)nocc[d], p.width[d],
+                               p.ioff[d], p.OPAD, cp32);
+            F2JK_LAUNCHED("pad_c32");
+        }
+    }
+    return F2JK_OK;
+}
+
+// D_d = Cp_d Cp_d^T via the contraction kernel with a single "P".
+int build_d(f2jk_ctx* ctx, const Plan& p, char* ws, hipStream_t st) {
+    const double* cp = (const double*)(ws + p.off_cp);
+    double* dbuf = (double*)(ws + p.off_d);
+    double* dpart = (double*)(ws + p.off_dpart);
+    const int nt = (int)cdiv(p.N, kTile);
+    const int nlower = nt * (nt + 1) / 2;
+    for (int d = 0; d < p.nd; ++d) {
+        hipLaunchKernelGGL(ks_kernel<double>, dim3(nlower), dim3(kGThreads), 0, st, cp, p.N, p.OPAD,
+                           p.ioff[d], p.width[d], 1, 1, nlower, dpart, 0);
+        F2JK_LAUNCHED("ks(D)");
+        hipLaunchKernelGGL(k_reduce_kernel, dim3((unsigned)cdiv(p.N, 256), p.N), dim3(256), 0, st,
+                           dpart, 1, p.N, dbuf + (size_t)d * p.N * p.N);
+        F2JK_LAUNCHED("k_reduce(D)");
+    }
+    return F2JK_OK;
+}
+
+// --------------------------------------------------------------------------- K
+template <typename T>
+int run_k_typed(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStream_t st) {
+    const T* B = (const T*)ctx->B[sizeof(T) == 8 ? F2JK_F64 : F2JK_F32];
+    const T* cp = (sizeof(T) == 8) ? (const T*)(ws + p.off_cp)
+                                   : (const T*)(ws + p.off_cp + align_up((size_t)p.N * p.OPAD * 8));
+    T* X = (T*)(ws + p.off_x);
+    double* kpart = (double*)(ws + p.off_kpart);
+    const int64_t NN = (int64_t)p.N * p.N;
+    for (int c = 0; c < p.nchunks; ++c) {
+        const int P0 = c * p.Pc;
+        const int pc = std::min(p.Pc, p.Q - P0);
+        const int64_t R = (int64_t)pc * p.N;
+        const int64_t nrowt = cdiv(R, kTile);
+        TIM(F2JK_PH_KX, true);
+        const T* A = B + (int64_t)P0 * NN;
+        const unsigned grid = (unsigned)(nrowt * p.ncolt);
+        if (p.kva == 4)
+            hipLaunchKernelGGL((kx_kernel<T, (sizeof(T) == 4 ? 4 : 2)>), dim3(grid), dim3(kGThreads), 0,
+                               st, A, R, p.N, cp, p.OPAD, X, p.ncolt);
+        else if (p.kva == 2)
+            hipLaunchKernelGGL((kx_kernel<T, 2>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp,
+                               p.OPAD, X, p.ncolt);
+        else
+            hipLaunchKernelGGL((kx_kernel<T, 1>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp,
+                               p.OPAD, X, p.ncolt);
+        F2JK_LAUNCHED("kx");
+        TIM(F2JK_PH_KX, false);
+        TIM(F2JK_PH_KS, true);
+        const int nsplit = std::min(p.nsplit, pc);
+        for (int d = 0; d < p.nd; ++d) {
+            hipLaunchKernelGGL(ks_kernel<T>, dim3(p.nlower * nsplit), dim3(kGThreads), 0, st, X, p.N,
+                               p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
+                               kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
+            F2JK_LAUNCHED("ks");
+        }
+        TIM(F2JK_PH_KS, false);
+    }
+    TIM(F2JK_PH_KRED, true);
+    const int nsplit_used = std::min(p.nsplit, std::min(p.Pc, p.Q));
+    for (int d = 0; d < p.nd; ++d) {
+        hipLaunchKernelGGL(k_reduce_kernel, dim3((unsigned)cdiv(p.N, 256), p.N), dim3(256), 0, st,
+                           kpart + (size_t)d * p.nsplit * NN, nsplit_used, p.N, K[d]);
+        F2JK_LAUNCHED("k_reduce");
+    }
+    TIM(F2JK_PH_KRED, false);
+    return F2JK_OK;
+}
+
+int check_ws(const Plan& p, void* ws, size_t ws_bytes) {
+    if (!ws) return fail(F2JK_EINVAL, "workspace pointer is NULL");
+    if (ws_bytes < p.total)
+        return fail(F2JK_ENOMEM, "workspace too small: need " + std::to_string(p.total) + " bytes, got " +
+                                     std::to_string(ws_bytes));
+    if (((uintptr_t)ws) % 256) return fail(F2JK_EINVAL, "workspace must be 256-byte aligned");
+    return F2JK_OK;
+}
+
+int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t* nocc,
+                 const int64_t* ldc, const double* const* Din, int64_t ldd, double* const* J,
+                 double* const* K, int precision, int flags, void* ws, size_t ws_bytes, void* stream,
+                 bool j_from_c) {
+    if (!ctx) return fail(F2JK_EINVAL, "null context");
+    Plan p;
+    int rc = make_plan(ctx, ndens, nocc, precision, flags, j_from_c, p);
+    if (rc) return rc;
+    if ((rc = check_ws(p, ws, ws_bytes))) return rc;
+    for (int d = 0; d < ndens; ++d) {
+        if (p.want_j && !J[d]) return fail(F2JK_EINVAL, "J output pointer is NULL");
+        if (p.want_k && !K[d]) return fail(F2JK_EINVAL, "K output pointer is NULL");
+        if (p.want_j && !j_from_c && !Din[d]) return fail(F2JK_EINVAL, "D pointer is NULL");
+    }
+    F2JK_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    char* w = (char*)ws;
+    tim_begin(ctx, flags & F2JK_TIMING);
+    TIM(F2JK_PH_TOTAL, true);
+    if (p.need_c) {
+        TIM(F2JK_PH_PREP, true);
+        if ((rc = pack_c(ctx, p, C, nocc, ldc, w, st, p.want_k && p.tk == 4))) return rc;
+        if (p.want_j && j_from_c)
+            if ((rc = build_d(ctx, p, w, st))) return rc;
+        TIM(F2JK_PH_PREP, false);
+    }
+    if (p.want_j) {
+        std::vector<const double*> dptr(ndens);
+        int64_t lddu = ldd;
+        for (int d = 0; d < ndens; ++d)
+            dptr[d] = j_from_c ? (const double*)(w + p.off_d) + (size_t)d * p.N * p.N : Din[d];
+        if (j_from_c) lddu = p.N;
+        if ((rc = run_j(ctx, p, dptr.data(), lddu, J, w, st))) return rc;
+    }
+    if (p.want_k) {
+        rc = (p.tk == 8) ? run_k_typed<double>(ctx, p, K, w, st) : run_k_typed<float>(ctx, p, K, w, st);
+        if (rc) return rc;
+    }
+    TIM(F2JK_PH_TOTAL, false);
+    g_last_error.clear();
+    return F2JK_OK;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+int f2jk_ctx_create(int device, f2jk_ctx** out) {
+    if (!out) return fail(F2JK_EINVAL, "out is NULL");
+    int ndev = 0;
+    F2JK_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(F2JK_EINVAL, "device index out of range");
+    F2JK_HIP(hipSetDevice(device));
+    f2jk_ctx* ctx = new f2jk_ctx();
+    ctx->device = device;
+    for (int i = 0; i < kMaxEvents + 2; ++i) {
+        hipEvent_t* slot = (i < kMaxEvents) ? &ctx->ev[i] : &ctx->ev_total[i - kMaxEvents];
+        hipError_t e = hipEventCreate(slot);
+        if (e != hipSuccess) {
+            for (int j = 0; j < i; ++j)
+                (void)hipEventDestroy(j < kMaxEvents ? ctx->ev[j] : ctx->ev_total[j - kMaxEvents]);
+            delete ctx;
+            return fail(F2JK_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = ctx;
+    return F2JK_OK;
+}
+
+int f2jk_ctx_destroy(f2jk_ctx* ctx) {
+    if (!ctx) return F2JK_OK;
+    (void)hipSetDevice(ctx->device);
+    for (int i = 0; i < kMaxEvents; ++i) (void)hipEventDestroy(ctx->ev[i]);
+    for (int i = 0; i < 2; ++i) (void)hipEventDestroy(ctx->ev_total[i]);
+    for (int v = 0; v < 5; ++v)
+        if (ctx->rowoff[v]) (void)hipFree(ctx->rowoff[v]);
+    delete ctx;
+    return F2JK_OK;
+}
+
+int f2jk_set_B(f2jk_ctx* ctx, int dtype, const void* B, int64_t naux, int64_t nbf) {
+    if (!ctx) return fail(F2JK_EINVAL, "null context");
+    if (dtype != F2JK_F64 && dtype != F2JK_F32) return fail(F2JK_EINVAL, "unknown dtype");
+    if (!B) return fail(F2JK_EINVAL, "B is NULL");
+    if (naux <= 0 || nbf <= 0 || naux > INT32_MAX || nbf > 65535)
+        return fail(F2JK_EINVAL, "naux/nbf out of range");
+    if (((uintptr_t)B) % 16) return fail(F2JK_EINVAL, "B must be 16-byte aligned");
+    const void* other = ctx->B[1 - dtype];
+    if (other && (ctx->naux != naux || ctx->nbf != nbf))
+        return fail(F2JK_EINVAL, "B copies must share naux and nbf");
+    if (ctx->nbf != nbf) {
+        (void)hipSetDevice(ctx->device);
+        for (int v = 0; v < 5; ++v)
+            if (ctx->rowoff[v]) {
+                (void)hipFree(ctx->rowoff[v]);
+                ctx->rowoff[v] = nullptr;
+                ctx->nvec[v] = 0;
+            }
+    }
+    ctx->B[dtype] = B;
+    ctx->naux = naux;
+    ctx->nbf = nbf;
+    // build the J element tables now so that builds allocate nothing
+    int rc = ensure_rowoff(ctx, j_vec_for(dtype, nbf));
+    if (rc) return rc;
+    g_last_error.clear();
+    return F2JK_OK;
+}
+
+int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf) {
+    if (!ctx) return fail(F2JK_EINVAL, "null context");
+    if (naux) *naux = ctx->naux;
+    if (nbf) *nbf = ctx->nbf;
+    return F2JK_OK;
+}
+
+int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
+    if (!ctx || !key) return fail(F2JK_EINVAL, "null argument");
+    if (!strcmp(key, "xchunk_bytes")) {
+        if (value < (1 << 20)) return fail(F2JK_EINVAL, "xchunk_bytes must be >= 1 MiB");
+        ctx->xchunk_bytes = (size_t)value;
+    } else if (!strcmp(key, "target_blocks")) {
+        if (value < 1 || value > (1 << 20)) return fail(F2JK_EINVAL, "target_blocks out of range");
+        ctx->target_blocks = (int)value;
+    } else {
+        return fail(F2JK_EINVAL, std::string("unknown tuning key ") + key);
+    }
+    return F2JK_OK;
+}
+
+int f2jk_workspace_size(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
+                        size_t* bytes) {
+    if (!ctx || !bytes) return fail(F2JK_EINVAL, "null argument");
+    Plan p;
+    // a build_jk plan (J from C) is the largest; use it whenever C is known
+    int rc = make_plan(ctx, ndens, nocc, precision, flags, nocc != nullptr, p);
+    if (rc) return rc;
+    *bytes = p.total;
+    return F2JK_OK;
+}
+
+int f2jk_build_jk(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t* nocc,
+                  const int64_t* ldc, double* const* J, double* const* K, int precision, int flags,
+                  void* ws, size_t ws_bytes, void* stream) {
+    if (!C || !nocc) return fail(F2JK_EINVAL, "C and nocc are required");
+    return build_common(ctx, ndens, C, nocc, ldc, nullptr, 0, J, K, precision, flags, ws, ws_bytes,
+                        stream, true);
+}
+
+int f2jk_build_j(f2jk_ctx* ctx, int ndens, const double* const* D, int64_t ldd, double* const* J,
+                 int precision, int flags, void* ws, size_t ws_bytes, void* stream) {
+    if (!D || !J) return fail(F2JK_EINVAL, "D and J are required");
+    if (ctx && ldd < ctx->nbf) return fail(F2JK_EINVAL, "ldd < nbf");
+    const int fl = (flags & F2JK_TIMING) | F2JK_WANT_J;
+    return build_common(ctx, ndens, nullptr, nullptr, nullptr, D, ldd, J, nullptr, precision, fl, ws,
+                        ws_bytes, stream, false);
+}
+
+int f2jk_build_k(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t* nocc,
+                 const int64_t* ldc, double* const* K, int precision, int flags, void* ws,
+                 size_t ws_bytes, void* stream) {
+    if (!C || !nocc || !K) return fail(F2JK_EINVAL, "C, nocc and K are required");
+    const int fl = (flags & F2JK_TIMING) | F2JK_WANT_K;
+    return build_common(ctx, ndens, C, nocc, ldc, nullptr, 0, nullptr, K, precision, fl, ws, ws_bytes,
+                        stream, false);
+}
+
+int f2jk_last_timings(f2jk_ctx* ctx, double* ms, int n) {
+    if (!ctx || !ms) return fail(F2JK_EINVAL, "null argument");
+    double acc[F2JK_NPHASE] = {0};
+    for (int i = 0; i + 1 < ctx->nev; i += 2) {
+        F2JK_HIP(hipEventSynchronize(ctx->ev[i + 1]));
+        float t = 0.f;
+        F2JK_HIP(hipEventElapsedTime(&t, ctx->ev[i], ctx->ev[i + 1]));
+        acc[ctx->ev_phase[i / 2]] += t;
+    }
+    if (ctx->total_recorded) {
+        F2JK_HIP(hipEventSynchronize(ctx->ev_total[1]));
+        float t = 0.f;
+        F2JK_HIP(hipEventElapsedTime(&t, ctx->ev_total[0], ctx->ev_total[1]));
+        acc[F2JK_PH_TOTAL] = t;
+    }
+    for (int i = 0; i < n && i < F2JK_NPHASE; ++i) ms[i] = acc[i];
+    return F2JK_OK;
+}
+
+int f2jk_work(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
+              double* j_bytes, double* k_flops) {
+    if (!ctx) return fail(F2JK_EINVAL, "null context");
+    const double N = (double)ctx->nbf, Q = (double)ctx->naux;
+    const double sb = (precision == F2JK_PREC_FP64) ? 8.0 : 4.0;
+    if (j_bytes) {
+        // two passes over the lower triangle of every B_P, plus D/J and gamma traffic
+        *j_bytes = (flags & F2JK_WANT_J)
+                       ? 2.0 * Q * (N * (N + 1) / 2) * sb + 8.0 * ndens * (2.0 * Q + 2.0 * N * N)
+                       : 0.0;
+    }
+    if (k_flops) {
+        double f = 0.0;
+        if ((flags & F2JK_WANT_K) && nocc)
+            for (int d = 0; d < ndens; ++d) {
+                const double o = (double)nocc[d];
+                f += 2.0 * Q * N * N * o + N * (N + 1) * Q * o;
+            }
+        *k_flops = f;
+    }
+    return F2JK_OK;
+}
+
+const char* f2jk_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,626 @@
+// Device kernels of the density-fitted J/K builder (gfx950 / CDNA4).
+//
+// Data layout in HBM
+//   B[P][m][n]   naux x nbf x nbf, row-major, symmetric in (m, n) -- the
+//                reference layout of FockBuilder.B_Pmn
+//                (forte2/jkbuilder/jkbuilder.py:52-72).
+//   Cp[n][i]     nbf x OPAD, occupied coefficients of all densities side by
+//                side, each density's block padded with zero columns to a
+//                multiple of 16 (the MFMA k step of the contraction).
+//   X[P][m][i]   half-transformed tensor of one P chunk, Pc x nbf x OPAD.
+//
+// J (two passes over the lower triangle of B, HBM-bound):
+//   pass 1  gamma_P = sum_{m>=n} w_mn B_Pmn D_mn,  w = 2 (m>n), 1 (m=n)
+//   pass 2  J_mn    = sum_P gamma_P B_Pmn           (m >= n, mirrored)
+//   The lower triangle is walked as "vector elements": row m is cut into
+//   ceil((m+1)/VEC) aligned VEC-wide pieces (16-byte loads), numbered
+//   row after row; rowoff[m] is the number of pieces in rows < m.
+// K (MFMA, compute-bound):
+//   KX  X = B_chunk Cp           one tall GEMM, (Pc*N) x N x OPAD
+//   KS  Kpart[s] += sum_{P in split s} X_P X_P^T   lower 128x128 tiles only
+//   KR  K = sum_s Kpart[s], lower triangle mirrored to the upper one
+// All reductions run in a fixed order (no atomics): the result of a build is
+// a function of its inputs and shapes only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace f2jk {
+
+constexpr int kWave = 64;
+constexpr int kJThreads = 256;
+constexpr int kJFlush = 32;  // P values reduced across waves per LDS flush
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Vec {
+    T v[V];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ Vec<T, V> load_vec(const T* p) {
+    return *reinterpret_cast<const Vec<T, V>*>(p);
+}
+
+template <int ND>
+struct DPtrs {
+    const double* p[ND];
+};
+template <int ND>
+struct JPtrs {
+    double* p[ND];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// largest m in [0, n) with rowoff[m] <= e  (rowoff[0] = 0, rowoff[n] = total)
+__device__ __forceinline__ int find_row(const int* __restrict__ rowoff, int n, int e) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (rowoff[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Give every XCD a contiguous range of logical tile ids so that tiles that
+// share operands run on the same L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+    constexpr int NX = 8;
+    const int xcd = b % NX, local = b / NX;
+    const int base = nblocks / NX, rem = nblocks % NX;
+    return xcd * base + (xcd < rem ? xcd : rem) + local;
+}
+
+// ---------------------------------------------------------------------------
+// J pass 1: partial gamma per (element group, density, P).
+// grid = (n_element_groups, ceil(Q / ppw)), block = 256.
+// ---------------------------------------------------------------------------
+template <typename TB, int VEC, int EPT, int ND>
+__global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
+    const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart) {
+    __shared__ double red[kJThreads / kWave][kJFlush][ND];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int64_t NN = (int64_t)N * N;
+
+    int64_t off[EPT];
+    double d[EPT][ND][VEC];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = blockIdx.x * (kJThreads * EPT) + k * kJThreads + tid;
+        off[k] = 0;
+#pragma unroll
+        for (int dd = 0; dd < ND; ++dd)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) d[k][dd][v] = 0.0;
+        if (e < nvec) {
+            const int m = find_row(rowoff, N, e);
+            const int col = (e - rowoff[m]) * VEC;
+            off[k] = (int64_t)m * N + col;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+                const int c = col + v;
+                const double w = (c < m) ? 2.0 : (c == m ? 1.0 : 0.0);
+#pragma unroll
+                for (int dd = 0; dd < ND; ++dd)
+                    d[k][dd][v] = (w != 0.0) ? w * D.p[dd][(int64_t)m * ldd + c] : 0.0;
+            }
+        }
+    }
+
+    const int P0 = blockIdx.y * ppw;
+    const int P1 = min(Q, P0 + ppw);
+    for (int Pb = P0; Pb < P1; Pb += kJFlush) {
+        const int np = min(kJFlush, P1 - Pb);
+        int j = 0;
+        for (; j + 1 < np; j += 2) {
+            const TB* B0 = B + (int64_t)(Pb + j) * NN;
+            const TB* B1 = B0 + NN;
+            Vec<TB, VEC> b0[EPT], b1[EPT];
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                b0[k] = load_vec<TB, VEC>(B0 + off[k]);
+                b1[k] = load_vec<TB, VEC>(B1 + off[k]);
+            }
+            double a0[ND], a1[ND];
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd) a0[dd] = a1[dd] = 0.0;
+#pragma unroll
+            for (int k = 0; k < EPT; ++k)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+#pragma unroll
+                    for (int dd = 0; dd < ND; ++dd) {
+                        a0[dd] = fma((double)b0[k].v[v], d[k][dd][v], a0[dd]);
+                        a1[dd] = fma((double)b1[k].v[v], d[k][dd][v], a1[dd]);
+                    }
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd) {
+                a0[dd] = wave_sum(a0[dd]);
+                a1[dd] = wave_sum(a1[dd]);
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int dd = 0; dd < ND; ++dd) {
+                    red[wid][j][dd] = a0[dd];
+                    red[wid][j + 1][dd] = a1[dd];
+                }
+            }
+        }
+        if (j < np) {
+            const TB* B0 = B + (int64_t)(Pb + j) * NN;
+            double a0[ND];
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd) a0[dd] = 0.0;
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                Vec<TB, VEC> b0 = load_vec<TB, VEC>(B0 + off[k]);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v)
+#pragma unroll
+                    for (int dd = 0; dd < ND; ++dd)
+                        a0[dd] = fma((double)b0.v[v], d[k][dd][v], a0[dd]);
+            }
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd) a0[dd] = wave_sum(a0[dd]);
+            if (lane == 0) {
+#pragma unroll
+                for (int dd = 0; dd < ND; ++dd) red[wid][j][dd] = a0[dd];
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < np * ND; t += kJThreads) {
+            const int jj = t / ND, dd = t % ND;
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < kJThreads / kWave; ++w) s += red[w][jj][dd];
+            gpart[((int64_t)blockIdx.x * ND + dd) * Q + Pb + jj] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// gamma[dd][P] = sum_g gpart[g][dd][P], g in a fixed order.
+__global__ void j_gamma_reduce_kernel(const double* __restrict__ gpart, int ngroups,
+                                      int ND, int Q, double* __restrict__ gamma) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ND * Q) return;
+    const int dd = (int)(t / Q), P = (int)(t % Q);
+    double s = 0.0;
+    for (int g = 0; g < ngroups; ++g) s += gpart[((int64_t)g * ND + dd) * Q + P];
+    gamma[t] = s;
+}
+
+// ---------------------------------------------------------------------------
+// J pass 2: partial J per (P chunk, density, element).
+// grid = (n_element_groups, n_p_chunks), block = 256.
+// ---------------------------------------------------------------------------
+template <typename TB, int VEC, int EPT, int ND>
+__global__ __launch_bounds__(kJThreads) void j_accum_kernel(
+    const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart) {
+    const int tid = threadIdx.x;
+    const int64_t NN = (int64_t)N * N;
+    int64_t off[EPT];
+    int eidx[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = blockIdx.x * (kJThreads * EPT) + k * kJThreads + tid;
+        eidx[k] = e;
+        off[k] = 0;
+        if (e < nvec) {
+            const int m = find_row(rowoff, N, e);
+            off[k] = (int64_t)m * N + (int64_t)(e - rowoff[m]) * VEC;
+        }
+    }
+    double acc[EPT][ND][VEC];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k)
+#pragma unroll
+        for (int dd = 0; dd < ND; ++dd)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) acc[k][dd][v] = 0.0;
+
+    const int P0 = blockIdx.y * ppw;
+    const int P1 = min(Q, P0 + ppw);
+    int P = P0;
+    for (; P + 1 < P1; P += 2) {
+        const TB* B0 = B + (int64_t)P * NN;
+        const TB* B1 = B0 + NN;
+        Vec<TB, VEC> b0[EPT], b1[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            b0[k] = load_vec<TB, VEC>(B0 + off[k]);
+            b1[k] = load_vec<TB, VEC>(B1 + off[k]);
+        }
+        double g0[ND], g1[ND];
+#pragma unroll
+        for (int dd = 0; dd < ND; ++dd) {
+            g0[dd] = gamma[(int64_t)dd * Q + P];
+            g1[dd] = gamma[(int64_t)dd * Q + P + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < EPT; ++k)
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) {
+                    acc[k][dd][v] = fma(g0[dd], (double)b0[k].v[v], acc[k][dd][v]);
+                    acc[k][dd][v] = fma(g1[dd], (double)b1[k].v[v], acc[k][dd][v]);
+                }
+    }
+    if (P < P1) {
+        const TB* B0 = B + (int64_t)P * NN;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            Vec<TB, VEC> b0 = load_vec<TB, VEC>(B0 + off[k]);
+#pragma unroll
+            for (int dd = 0; dd < ND; ++dd) {
+                const double g = gamma[(int64_t)dd * Q + P];
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) acc[k][dd][v] = fma(g, (double)b0.v[v], acc[k][dd][v]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        if (eidx[k] >= nvec) continue;
+#pragma unroll
+        for (int dd = 0; dd < ND; ++dd) {
+            double* dst = jpart + (((int64_t)blockIdx.y * ND + dd) * nvec + eidx[k]) * VEC;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) dst[v] = acc[k][dd][v];
+        }
+    }
+}
+
+// J[dd][m][c] = J[dd][c][m] = sum_chunk jpart[chunk][dd][e][v]  (c = col+v <= m)
+template <int VEC, int ND>
+__global__ void j_reduce_kernel(const double* __restrict__ jpart, int nchunks, int N,
+                                const int* __restrict__ rowoff, int nvec, JPtrs<ND> J) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nvec) return;
+    const int m = find_row(rowoff, N, e);
+    const int col = (e - rowoff[m]) * VEC;
+#pragma unroll
+    for (int dd = 0; dd < ND; ++dd) {
+        double s[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) s[v] = 0.0;
+        for (int c = 0; c < nchunks; ++c) {
+            const double* src = jpart + (((int64_t)c * ND + dd) * nvec + e) * VEC;
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) s[v] += src[v];
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+            const int cc = col + v;
+            if (cc <= m) {
+                J.p[dd][(int64_t)m * N + cc] = s[v];
+                J.p[dd][(int64_t)cc * N + m] = s[v];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA helpers.  v_mfma_{f64,f32}_16x16x4: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  Accumulator element r of lane l holds D[i][l%16] with
+//   f32:  i = 4*(l/16) + r
+//   f64:  i = (l/16) + 4*r          (measured on gfx950)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int acc_row(int lane, int r) {
+    if constexpr (sizeof(T) == 8)
+        return (lane >> 4) + 4 * r;
+    else
+        return 4 * (lane >> 4) + r;
+}
+
+__device__ __forceinline__ dx4 mfma4(double a, double b, dx4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ fx4 mfma4(float a, float b, fx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+    typedef dx4 type;
+};
+template <>
+struct Acc<float> {
+    typedef fx4 type;
+};
+
+constexpr int kTile = 128;   // output tile edge (rows and columns)
+constexpr int kBK = 16;      // k extent of one LDS stage
+constexpr int kGThreads = 256;
+
+// ---------------------------------------------------------------------------
+// KX:  X[r][c] = sum_k A[r][k] Cp[k][c]
+//   A  = B of the chunk viewed as R x K row-major (R = Pc*N rows, K = N)
+//   Cp = K x OPAD row-major (zero-padded columns), X = R x OPAD row-major.
+// Tile 128 x 128, 4 waves as 2 x 2 blocks of 64 x 64 (4 x 4 MFMA blocks each).
+// VA: elements per global load of A (16 B when the row length allows).
+// ---------------------------------------------------------------------------
+template <typename T, int VA>
+__global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, int64_t R,
+                                                       int K, const T* __restrict__ Cp,
+                                                       int OPAD, T* __restrict__ X,
+                                                       int ncolt) {
+    typedef typename Acc<T>::type acc_t;
+    constexpr int VB = 16 / sizeof(T);       // B tile loads are always 16 B (OPAD % 16 == 0)
+    __shared__ T As[kTile][kBK + 1];
+    __shared__ T Bs[kBK][kTile];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t r0 = (int64_t)(t / ncolt) * kTile;
+    const int c0 = (t % ncolt) * kTile;
+    const int wr = wid >> 1, wc = wid & 1;
+    const int wcol0 = c0 + wc * 64;
+    int nbv = (OPAD - wcol0) / 16;
+    nbv = nbv < 0 ? 0 : (nbv > 4 ? 4 : nbv);
+
+    constexpr int A_LD = (kTile * kBK) / (kGThreads * VA);  // loads per thread
+    constexpr int B_LD = (kTile * kBK) / (kGThreads * VB);
+    Vec<T, VA> ra[A_LD];
+    Vec<T, VB> rb[B_LD];
+
+    auto load_tiles = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
+            const int64_t gr = r0 + row;
+            const int gk = k0 + kc;
+            if (gr < R && gk < K) {
+                ra[s] = load_vec<T, VA>(A + gr * K + gk);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VA; ++v) ra[s].v[v] = T(0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < B_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int kr = idx / (kTile / VB), cc = (idx % (kTile / VB)) * VB;
+            const int gk = k0 + kr, gc = c0 + cc;
+            if (gk < K && gc < OPAD) {
+                rb[s] = load_vec<T, VB>(Cp + (int64_t)gk * OPAD + gc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VB; ++v) rb[s].v[v] = T(0);
+            }
+        }
+    };
+    auto store_tiles = [&]() {
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
+#pragma unroll
+            for (int v = 0; v < VA; ++v) As[row][kc + v] = ra[s].v[v];
+        }
+#pragma unroll
+        for (int s = 0; s < B_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int kr = idx / (kTile / VB), cc = (idx % (kTile / VB)) * VB;
+#pragma unroll
+            for (int v = 0; v < VB; ++v) Bs[kr][cc + v] = rb[s].v[v];
+        }
+    };
+
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    const int nk = (K + kBK - 1) / kBK;
+    load_tiles(0);
+    for (int kt = 0; kt < nk; ++kt) {
+        store_tiles();
+        __syncthreads();
+        if (kt + 1 < nk) load_tiles((kt + 1) * kBK);
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 4) {
+            T a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < nbv) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i][j] = mfma4(a[i], b[j], acc[i][j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nbv) continue;
+            const int col = wcol0 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + wr * 64 + i * 16 + acc_row<T>(lane, r);
+                if (row < R) X[row * OPAD + col] = acc[i][j][r];
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// KS:  Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
+// Only lower tiles (mt >= nt) are computed.  grid = nlower * nsplit.
+// FP32 inputs accumulate each P in FP32 and fold it into FP64 registers.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, int N, int OPAD,
+                                                       int ioff, int width, int Pc, int nsplit,
+                                                       int nlower, double* __restrict__ Kpart,
+                                                       int accumulate) {
+    typedef typename Acc<T>::type acc_t;
+    constexpr int VA = 16 / sizeof(T);
+    __shared__ T As[kTile][kBK + 1];
+    __shared__ T Bs[kTile][kBK + 1];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = lt / nlower;
+    int tl = lt % nlower;
+    int mt = 0;
+    while ((mt + 1) * (mt + 2) / 2 <= tl) ++mt;
+    const int nt = tl - mt * (mt + 1) / 2;
+    const int m0 = mt * kTile, n0 = nt * kTile;
+    const int wr = wid >> 1, wc = wid & 1;
+    const int Pa = (int)(((int64_t)Pc * s) / nsplit);
+    const int Pb = (int)(((int64_t)Pc * (s + 1)) / nsplit);
+
+    constexpr int LD = (kTile * kBK) / (kGThreads * VA);
+    Vec<T, VA> ra[LD], rb[LD];
+
+    auto load_tiles = [&](int P, int i0) {
+        const T* XP = X + (int64_t)P * N * OPAD + ioff + i0;
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            const int idx = tid + kGThreads * q;
+            const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
+            if (m0 + row < N) {
+                ra[q] = load_vec<T, VA>(XP + (int64_t)(m0 + row) * OPAD + kc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VA; ++v) ra[q].v[v] = T(0);
+            }
+            if (n0 + row < N) {
+                rb[q] = load_vec<T, VA>(XP + (int64_t)(n0 + row) * OPAD + kc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VA; ++v) rb[q].v[v] = T(0);
+            }
+        }
+    };
+    auto store_tiles = [&]() {
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            const int idx = tid + kGThreads * q;
+            const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
+#pragma unroll
+            for (int v = 0; v < VA; ++v) {
+                As[row][kc + v] = ra[q].v[v];
+                Bs[row][kc + v] = rb[q].v[v];
+            }
+        }
+    };
+
+    dx4 acc64[4][4];
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc64[i][j] = dx4{0, 0, 0, 0};
+            acc[i][j] = acc_t{0, 0, 0, 0};
+        }
+
+    const int nkp = width / kBK;  // k steps per P (width % 16 == 0)
+    const int nsteps = (Pb - Pa) * nkp;
+    if (nsteps > 0) load_tiles(Pa, 0);
+    for (int st = 0; st < nsteps; ++st) {
+        store_tiles();
+        __syncthreads();
+        if (st + 1 < nsteps) {
+            const int sn = st + 1;
+            load_tiles(Pa + sn / nkp, (sn % nkp) * kBK);
+        }
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 4) {
+            T a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[wc * 64 + j * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+        if constexpr (sizeof(T) == 4) {
+            if ((st + 1) % nkp == 0) {  // end of one P: fold FP32 partials into FP64
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
+                        acc[i][j] = acc_t{0, 0, 0, 0};
+                    }
+            }
+        }
+    }
+    auto outval = [&](int i, int j, int r) -> double {
+        if constexpr (sizeof(T) == 8)
+            return acc[i][j][r];
+        else
+            return acc64[i][j][r];
+    };
+    double* Ks = Kpart + (int64_t)s * N * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 64 + i * 16 + acc_row<T>(lane, r);
+                if (row >= N) continue;
+                double* dst = Ks + (int64_t)row * N + col;
+                const double v = outval(i, j, r);
+                *dst = accumulate ? (*dst + v) : v;
+            }
+        }
+}
+
+// K[m][n] = K[n][m] = sum_s Kpart[s][m][n] for m >= n (fixed order in s).
+__global__ void k_reduce_kernel(const double* __restrict__ Kpart, int nsplit, int N,
+                                double* __restrict__ K) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (n > m || m >= N) return;
+    const int64_t NN = (int64_t)N * N;
+    double acc = 0.0;
+    for (int s = 0; s < nsplit; ++s) acc += Kpart[s * NN + (int64_t)m * N + n];
+    K[(int64_t)m * N + n] = acc;
+    K[(int64_t)n * N + m] = acc;
+}
+
+// Cp[n][ioff + i] = C[n][i] (i < nocc), zero for nocc <= i < width.
+template <typename T>
+__global__ void pad_c_kernel(const double* __restrict__ C, int64_t ldc, int N, int nocc,
+                             int width, int ioff, int OPAD, T* __restrict__ Cp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)N * width) return;
+    const int n = (int)(t / width), i = (int)(t % width);
+    Cp[(int64_t)n * OPAD + ioff + i] = (i < nocc) ? (T)C[(int64_t)n * ldc + i] : T(0);
+}
+
+}  // namespace f2jk

@@ -1,0 +1,298 @@
+"""Device-resident density-fitted J/K builder (drop-in for forte2's FockBuilder).
+
+Mirrors the duck-typed surface of ``forte2.jkbuilder.FockBuilder``
+(forte2/jkbuilder/jkbuilder.py:21-595) that the RHF/UHF/MCSCF loops call:
+``build_JK``, ``build_J``, ``build_K``, ``build_JK_generalized``,
+``build_core_fock``, ``build_active_fock``, ``build_generalized_fock`` and the
+attributes ``system``, ``nbf``, ``naux``, ``B_Pmn``.
+
+The 3-index tensor B(P|mn) lives in HBM as a torch tensor and every
+contraction runs in the hand-written HIP kernels of ``libf2jk.so`` through its
+C ABI.  Inputs may be numpy arrays (results come back as numpy, like the
+reference) or torch tensors on the GPU (results stay on the device).
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def _device_of(device):
+    if device is None:
+        return torch.device("cuda", torch.cuda.current_device())
+    d = torch.device(device)
+    if d.type != "cuda":
+        raise ValueError("FockBuilder runs on a HIP device only")
+    if d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return d
+
+
+class FockBuilder:
+    """J/K builder with B(P|mn) resident in HBM.
+
+    Parameters
+    ----------
+    system : System, optional
+        Molecule and basis sets; B is built from it on first use
+        (forte2/jkbuilder/jkbuilder.py:52-72, 119-159).
+    store_B_nPm : bool
+        Accepted for signature compatibility with the reference
+        (forte2/jkbuilder/jkbuilder.py:46-50).  The device kernels read the
+        (P, m, n) layout directly, so no transposed copy is ever made.
+    B : array or tensor (naux, nbf, nbf), optional
+        A ready B tensor (e.g. from a previous run); copied to the device.
+    precision : {"fp64", "fp32", "bf16x3"}
+        Arithmetic of ``build_*`` calls (see include/f2jk.h).  fp32/bf16x3
+        create an FP32 copy of B next to the FP64 one.
+    """
+
+    def __init__(self, system=None, store_B_nPm=True, *, B=None, precision="fp64", device=None):
+        self.system = system
+        self.store_B_nPm = store_B_nPm
+        self.device = _device_of(device)
+        self._B64 = None
+        self._B32 = None
+        self._ctx = None
+        self._ws = None
+        self.timing = False
+        self.last_timings = None
+        if precision not in nat.PREC:
+            raise ValueError(f"unknown precision {precision!r}")
+        self.precision = precision
+        if B is not None:
+            self._set_B(B)
+        elif system is not None:
+            self.nbf = system.nbf
+        else:
+            raise ValueError("FockBuilder needs a system or a B tensor")
+
+    # ------------------------------------------------------------------ B
+    def _set_B(self, B):
+        if isinstance(B, np.ndarray):
+            B = torch.from_numpy(np.ascontiguousarray(B, dtype=np.float64))
+        B = B.to(device=self.device, dtype=torch.float64).contiguous()
+        if B.dim() != 3 or B.shape[1] != B.shape[2]:
+            raise ValueError("B must have shape (naux, nbf, nbf)")
+        self._B64 = B
+        self._B32 = None
+        self.naux, self.nbf = int(B.shape[0]), int(B.shape[1])
+        if self._ctx is not None:
+            self._register(nat.F2JK_F64, self._B64)
+
+    @property
+    def B_Pmn(self):
+        """B(P|mn) as a device tensor (built lazily from the system)."""
+        if self._B64 is None:
+            from .df import build_B_density_fitting
+
+            self._set_B(build_B_density_fitting(self.system, device=self.device))
+        return self._B64
+
+    def B_fp32(self):
+        """The FP32 copy of B used by the fp32/bf16x3 precision forms."""
+        if self._B32 is None:
+            self._B32 = self.B_Pmn.to(torch.float32)
+            if self._ctx is not None:
+                self._register(nat.F2JK_F32, self._B32)
+        return self._B32
+
+    # ------------------------------------------------------------------ native context
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            lib = nat.jk_lib()
+            h = ctypes.c_void_p()
+            nat.jk_check(lib.f2jk_ctx_create(self.device.index, ctypes.byref(h)), "f2jk_ctx_create")
+            self._ctx = h
+            self._register(nat.F2JK_F64, self.B_Pmn)
+            if self._B32 is not None:
+                self._register(nat.F2JK_F32, self._B32)
+        return self._ctx
+
+    def _register(self, dtype, B):
+        lib = nat.jk_lib()
+        nat.jk_check(
+            lib.f2jk_set_B(self._ctx, dtype, ctypes.c_void_p(B.data_ptr()), B.shape[0], B.shape[1]),
+            "f2jk_set_B",
+        )
+
+    def set_tuning(self, key, value):
+        nat.jk_check(nat.jk_lib().f2jk_set_tuning(self.ctx, key.encode(), int(value)), "f2jk_set_tuning")
+
+    def __del__(self):
+        try:
+            if self._ctx is not None:
+                nat.jk_lib().f2jk_ctx_destroy(self._ctx)
+        except Exception:
+            pass
+
+    def _workspace(self, nbytes):
+        if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def _prec(self, precision):
+        p = self.precision if precision is None else precision
+        if p not in nat.PREC:
+            raise ValueError(f"unknown precision {p!r}")
+        if p != "fp64":
+            self.B_fp32()
+        return nat.PREC[p]
+
+    # ------------------------------------------------------------------ helpers
+    def _as_dev(self, x):
+        if isinstance(x, torch.Tensor):
+            return x.to(device=self.device, dtype=torch.float64), False
+        return torch.from_numpy(np.asarray(x, dtype=np.float64)).to(self.device), True
+
+    @staticmethod
+    def _rowmajor(x):
+        """(tensor, leading dimension) with unit column stride."""
+        if x.dim() != 2:
+            raise ValueError("expected a 2-D matrix")
+        if x.stride(1) != 1 or x.stride(0) < max(1, x.shape[1]):
+            x = x.contiguous()
+        return x, max(int(x.stride(0)), 1)
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _collect_timings(self):
+        if self.timing:
+            ms = (ctypes.c_double * len(nat.PHASES))()
+            nat.jk_check(nat.jk_lib().f2jk_last_timings(self.ctx, ms, len(nat.PHASES)), "timings")
+            self.last_timings = dict(zip(nat.PHASES, list(ms)))
+
+    # ------------------------------------------------------------------ builds
+    def _jk_call(self, C, want_j, want_k, precision=None):
+        if not isinstance(C, (list, tuple)):
+            raise TypeError("C must be a list of coefficient matrices")
+        ctx = self.ctx
+        prec = self._prec(precision)
+        N = self.nbf
+        to_np = False
+        Cs, nocc, ldc = [], [], []
+        for Ci in C:
+            t, was_np = self._as_dev(Ci)
+            to_np = to_np or was_np
+            if t.dim() != 2 or t.shape[0] != N:
+                raise ValueError(f"C must have {N} rows")
+            t, ld = self._rowmajor(t)
+            Cs.append(t)
+            nocc.append(int(t.shape[1]))
+            ldc.append(ld)
+        nd = len(Cs)
+        J = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in range(nd)] if want_j else []
+        K = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in range(nd)] if want_k else []
+        flags = (nat.F2JK_WANT_J if want_j else 0) | (nat.F2JK_WANT_K if want_k else 0)
+        if self.timing:
+            flags |= nat.F2JK_TIMING
+        lib = nat.jk_lib()
+        nocc_a = nat.i64_array(nocc)
+        nbytes = ctypes.c_size_t()
+        nat.jk_check(lib.f2jk_workspace_size(ctx, nd, nocc_a, prec, flags, ctypes.byref(nbytes)), "workspace")
+        ws = self._workspace(nbytes.value)
+        cp = nat.ptr_array([c.data_ptr() for c in Cs])
+        jp = nat.ptr_array([j.data_ptr() for j in J]) if want_j else None
+        kp = nat.ptr_array([k.data_ptr() for k in K]) if want_k else None
+        ldc_a = nat.i64_array(ldc)
+        if want_j:
+            rc = lib.f2jk_build_jk(ctx, nd, cp, nocc_a, ldc_a, jp, kp, prec, flags,
+                                   ctypes.c_void_p(ws.data_ptr()), ws.numel(), self._stream())
+            nat.jk_check(rc, "f2jk_build_jk")
+        else:
+            rc = lib.f2jk_build_k(ctx, nd, cp, nocc_a, ldc_a, kp, prec, flags,
+                                  ctypes.c_void_p(ws.data_ptr()), ws.numel(), self._stream())
+            nat.jk_check(rc, "f2jk_build_k")
+        self._collect_timings()
+        if to_np:
+            J = [j.cpu().numpy() for j in J]
+            K = [k.cpu().numpy() for k in K]
+        return J, K
+
+    def build_J(self, D, precision=None):
+        """J_k = sum_P B_P <B_P, D_k> for each D_k (jkbuilder.py:161-166)."""
+        if not isinstance(D, (list, tuple)):
+            raise TypeError("D must be a list of density matrices")
+        ctx = self.ctx
+        prec = self._prec(precision)
+        N = self.nbf
+        to_np = False
+        Ds = []
+        for Di in D:
+            t, was_np = self._as_dev(Di)
+            to_np = to_np or was_np
+            if t.shape != (N, N):
+                raise ValueError(f"D must be {N}x{N}")
+            Ds.append(t.contiguous())
+        nd = len(Ds)
+        J = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in range(nd)]
+        flags = nat.F2JK_WANT_J | (nat.F2JK_TIMING if self.timing else 0)
+        lib = nat.jk_lib()
+        nbytes = ctypes.c_size_t()
+        nat.jk_check(lib.f2jk_workspace_size(ctx, nd, None, prec, flags, ctypes.byref(nbytes)), "workspace")
+        ws = self._workspace(nbytes.value)
+        rc = lib.f2jk_build_j(ctx, nd, nat.ptr_array([d.data_ptr() for d in Ds]), N,
+                              nat.ptr_array([j.data_ptr() for j in J]), prec, flags,
+                              ctypes.c_void_p(ws.data_ptr()), ws.numel(), self._stream())
+        nat.jk_check(rc, "f2jk_build_j")
+        self._collect_timings()
+        return [j.cpu().numpy() for j in J] if to_np else J
+
+    def build_K(self, C, precision=None):
+        """K_k = sum_{P,i} (B_P C_k)_mi (B_P C_k)_ni (jkbuilder.py:184-207)."""
+        return self._jk_call(C, False, True, precision)[1]
+
+    def build_JK(self, C, precision=None):
+        """(J_k, K_k) for each C_k with D_k = C_k C_k^T (jkbuilder.py:209-251)."""
+        return self._jk_call(C, True, True, precision)
+
+    # ------------------------------------------------------------------ generalized Fock
+    def build_JK_generalized(self, C, g1, precision=None):
+        """J/K of the density C g1 C^T via a factor of g1 (jkbuilder.py:253-290)."""
+        t, to_np = self._as_dev(C)
+        g, _ = self._as_dev(g1)
+        if t.shape[1] != g.shape[0]:
+            raise AssertionError("C and g1 must have compatible dimensions")
+        try:
+            L = torch.linalg.cholesky(g)
+            Cp = t @ L
+        except RuntimeError:
+            n, U = torch.linalg.eigh(g)
+            if bool((n < -1.0e-11).any()):
+                raise AssertionError("g1 must be positive semi-definite")
+            Cp = t @ U @ torch.diag(torch.sqrt(torch.clamp(n, min=0.0)))
+        J, K = self._jk_call([Cp], True, True, precision)
+        if to_np:
+            return J[0].cpu().numpy(), K[0].cpu().numpy()
+        return J[0], K[0]
+
+    def build_core_fock(self, C_core, hcore=None, precision=None):
+        """F_core = h + 2 J[C_core] - K[C_core] (jkbuilder.py:292-321)."""
+        if hcore is None:
+            hcore = self.system.ints_hcore()
+        J, K = self.build_JK([C_core], precision)
+        return hcore + 2.0 * J[0] - K[0]
+
+    def build_active_fock(self, C_act, g1, precision=None):
+        """F_act = J[g1] - K[g1] / 2 (jkbuilder.py:323-349)."""
+        J, K = self.build_JK_generalized(C_act, g1, precision)
+        return J - 0.5 * K
+
+    def build_generalized_fock(self, C_core, C_act, g1, hcore=None, precision=None):
+        """F = F_core + F_act (jkbuilder.py:351-380)."""
+        return self.build_core_fock(C_core, hcore, precision) + self.build_active_fock(C_act, g1, precision)
+
+    # ------------------------------------------------------------------ measurement
+    def work(self, nocc, precision="fp64", want_j=True, want_k=True):
+        """(J bytes, K flops) of one build: Bytes_J = 2 Q N(N+1)/2 s_B + 8 nd (2Q + 2N^2),
+        Flops_K = sum_k 2 Q N^2 o_k + N(N+1) Q o_k."""
+        jb, kf = ctypes.c_double(), ctypes.c_double()
+        flags = (nat.F2JK_WANT_J if want_j else 0) | (nat.F2JK_WANT_K if want_k else 0)
+        nat.jk_check(nat.jk_lib().f2jk_work(self.ctx, len(nocc), nat.i64_array(nocc), nat.PREC[precision],
+                                            flags, ctypes.byref(jb), ctypes.byref(kf)), "f2jk_work")
+        return jb.value, kf.value

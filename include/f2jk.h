@@ -1,0 +1,136 @@
+/*
+ * f2jk.h -- C ABI of the MI355X density-fitted J/K builder (libf2jk.so).
+ *
+ * Replaces the numpy/BLAS body of forte2's FockBuilder:
+ *   forte2/jkbuilder/jkbuilder.py:209-251  FockBuilder.build_JK(C)  -> f2jk_build_jk
+ *   forte2/jkbuilder/jkbuilder.py:161-166  FockBuilder.build_J(D)   -> f2jk_build_j
+ *   forte2/jkbuilder/jkbuilder.py:184-207  FockBuilder.build_K(C)   -> f2jk_build_k
+ *   forte2/jkbuilder/jkbuilder.py:52-80    cached B_Pmn / B_nPm     -> f2jk_set_B (borrowed, resident in HBM)
+ *
+ * Conventions
+ *   - B is the density-fitted 3-index tensor B[P][m][n] (naux x nbf x nbf,
+ *     row-major, symmetric in m,n), exactly the reference's B_Pmn layout.
+ *     The library never copies or frees it; the caller keeps it alive.
+ *   - C_k is an nbf x nocc_k row-major FP64 matrix with row stride ldc_k
+ *     (a column slice C[:, :nocc] of an nbf x nmo array has ldc = nmo).
+ *   - D, J, K are nbf x nbf row-major FP64 (D with row stride ldd, J/K dense).
+ *   - All pointers are device pointers.  Every call is stream-ordered on the
+ *     given hipStream_t (NULL = default stream) and never synchronises.
+ *   - Scratch memory is supplied by the caller (ws, ws_bytes); query its size
+ *     with f2jk_workspace_size.  The library allocates device memory only in
+ *     f2jk_ctx_create / f2jk_set_B (small index tables).
+ *   - Reductions are deterministic: a result depends only on the inputs and
+ *     the context's shapes, never on concurrent work.
+ *   - Return value: 0 on success, a negative f2jk_status otherwise; the
+ *     message of the last failure on the calling thread is f2jk_last_error().
+ */
+#ifndef F2JK_H
+#define F2JK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct f2jk_ctx f2jk_ctx;
+
+enum f2jk_status {
+    F2JK_OK = 0,
+    F2JK_EINVAL = -1,      /* bad argument (shape, pointer, enum)            */
+    F2JK_EHIP = -2,        /* a HIP runtime call or kernel launch failed     */
+    F2JK_ENOMEM = -3,      /* workspace too small / allocation failed        */
+    F2JK_EUNSUPPORTED = -4,/* combination not implemented                    */
+    F2JK_ESTATE = -5       /* context not ready (B not set for this dtype)   */
+};
+
+/* storage type of a B copy held by the context */
+enum f2jk_dtype { F2JK_F64 = 0, F2JK_F32 = 1 };
+
+/* arithmetic of a build.  FP64: B64, FP64 MFMA, FP64 accumulation.
+ * FP32: B32 for J (FP64 accumulation) and FP32 MFMA for K (FP32 accumulation
+ * within a P block, FP64 across blocks).
+ * BF16X3: B32 for J; K operands split into bf16 hi/lo pairs, three bf16 MFMA
+ * products (hi*hi + hi*lo + lo*hi) with FP32 accumulation. */
+enum f2jk_precision { F2JK_PREC_FP64 = 0, F2JK_PREC_FP32 = 1, F2JK_PREC_BF16X3 = 2 };
+
+/* what f2jk_build_jk computes; F2JK_TIMING records per-phase HIP events */
+enum f2jk_flags { F2JK_WANT_J = 1, F2JK_WANT_K = 2, F2JK_TIMING = 16 };
+
+/* phases reported by f2jk_last_timings (milliseconds, in this order) */
+enum f2jk_phase {
+    F2JK_PH_PREP = 0,      /* C padding, D = C C^T                           */
+    F2JK_PH_JGAMMA = 1,    /* J pass 1: gamma_P = sum_mn B_Pmn D_mn          */
+    F2JK_PH_JACC = 2,      /* J pass 2: J_mn = sum_P gamma_P B_Pmn (+reduce) */
+    F2JK_PH_KX = 3,        /* K half-transform X_Pmi = sum_n B_Pmn C_ni      */
+    F2JK_PH_KS = 4,        /* K contraction K_mn += sum_Pi X_Pmi X_Pni       */
+    F2JK_PH_KRED = 5,      /* K split reduction + mirror                     */
+    F2JK_PH_TOTAL = 6,
+    F2JK_NPHASE = 7
+};
+
+/* Create a context on HIP device `device`.  Replaces the construction of
+ * FockBuilder (forte2/jkbuilder/jkbuilder.py:46-50). */
+int f2jk_ctx_create(int device, f2jk_ctx** out);
+int f2jk_ctx_destroy(f2jk_ctx* ctx);
+
+/* Register a resident copy of B (borrowed device pointer) of the given dtype.
+ * A context may hold one F64 and one F32 copy of the same tensor; both must
+ * have the same naux and nbf.  Replaces FockBuilder.B_Pmn
+ * (forte2/jkbuilder/jkbuilder.py:52-72) and B_nPm (:74-80); the transposed
+ * copy is not needed. */
+int f2jk_set_B(f2jk_ctx* ctx, int dtype, const void* B, int64_t naux, int64_t nbf);
+
+/* Shapes of the registered B. */
+int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
+
+/* Launch-shape knobs (no counterpart in the reference, whose BLAS calls have
+ * none): "xchunk_bytes" caps the half-transformed chunk X held in the
+ * workspace (default 4 GiB); "target_blocks" is the number of workgroups a
+ * split launch aims for (default 2048).  Set by the tile autotuner. */
+int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value);
+
+/* Scratch bytes needed by f2jk_build_jk / _j / _k for these arguments.
+ * nocc may be NULL when flags has no F2JK_WANT_K and no F2JK_WANT_J with C. */
+int f2jk_workspace_size(const f2jk_ctx* ctx, int ndens, const int64_t* nocc,
+                        int precision, int flags, size_t* bytes);
+
+/* build_JK (forte2/jkbuilder/jkbuilder.py:209-251): for k < ndens,
+ *   D_k = C_k C_k^T, J_k = J[D_k], K_k = sum_P (B_P C_k)(B_P C_k)^T.
+ * J or K may be NULL when the corresponding flag is absent. */
+int f2jk_build_jk(f2jk_ctx* ctx, int ndens, const double* const* C,
+                  const int64_t* nocc, const int64_t* ldc, double* const* J,
+                  double* const* K, int precision, int flags, void* ws,
+                  size_t ws_bytes, void* stream);
+
+/* build_J (forte2/jkbuilder/jkbuilder.py:161-166): J_k = sum_P B_P <B_P, D_k>
+ * for symmetric D_k (only the lower triangle of D_k is read). */
+int f2jk_build_j(f2jk_ctx* ctx, int ndens, const double* const* D, int64_t ldd,
+                 double* const* J, int precision, int flags, void* ws,
+                 size_t ws_bytes, void* stream);
+
+/* build_K (forte2/jkbuilder/jkbuilder.py:184-207, the B_nPm path):
+ * K_k = sum_{P,i} X_Pmi X_Pni with X_P = B_P C_k. */
+int f2jk_build_k(f2jk_ctx* ctx, int ndens, const double* const* C,
+                 const int64_t* nocc, const int64_t* ldc, double* const* K,
+                 int precision, int flags, void* ws, size_t ws_bytes,
+                 void* stream);
+
+/* Per-phase times (ms) of the last build issued with F2JK_TIMING; waits for
+ * that build's events.  n <= F2JK_NPHASE values are written. */
+int f2jk_last_timings(f2jk_ctx* ctx, double* ms, int n);
+
+/* Algorithmic work of one build with these arguments (for roofline
+ * reporting): bytes of B streamed by the J passes and K flops
+ * (2*Q*N^2*o half-transform + N(N+1)*Q*o symmetric contraction). */
+int f2jk_work(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision,
+              int flags, double* j_bytes, double* k_flops);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* f2jk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* F2JK_H */

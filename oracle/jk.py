@@ -1,0 +1,82 @@
+"""numpy FP64 restatement of forte2's FockBuilder algebra (test infrastructure).
+
+Pinned by tests/test_oracle_jk.py: every function is checked against an
+explicit 4-index brute force on small random inputs ((mn|rs) = sum_P B_Pmn
+B_Prs contracted by plain Python loops), against linearity / symmetry /
+positivity invariants, and -- through the SCF oracle -- against the reference's
+golden energies (tests/test_oracle_scf.py).
+"""
+
+import numpy as np
+import scipy.linalg
+
+
+def build_B_density_fitting(M, Pmn):
+    """B = L^-1 (Q|mn) with M = L L^T (forte2/jkbuilder/jkbuilder.py:135-159).
+
+    Raises ValueError when the metric is not positive definite (:141-148).
+    """
+    try:
+        L = scipy.linalg.cholesky(M, lower=True)
+    except scipy.linalg.LinAlgError:
+        raise ValueError("Density fitting Coulomb metric (P|Q) is not positive definite.")
+    naux = M.shape[0]
+    X = scipy.linalg.solve_triangular(L, np.eye(naux), lower=True)
+    return np.einsum("PQ,Qmn->Pmn", X, Pmn, optimize=True)
+
+
+def build_J(B, D):
+    """[sum_P B_Pmn sum_rs B_Prs D_sr for D_k in D] (jkbuilder.py:161-166)."""
+    return [np.einsum("Pmn,Prs,sr->mn", B, B, Di, optimize=True) for Di in D]
+
+
+def build_K_Pmn(B, C):
+    """K via the (P,m,n) layout (jkbuilder.py:168-182, real case)."""
+    Y = [np.einsum("Pms,si->Pmi", B, Ci, optimize=True) for Ci in C]
+    return [np.einsum("Pmi,Pni->mn", Yi, Yi, optimize=True) for Yi in Y]
+
+
+def build_K_nPm(B, C, BnPm=None):
+    """K via the transposed (n,P,m) copy, the reference default (jkbuilder.py:74-80, 184-201)."""
+    if BnPm is None:
+        BnPm = np.ascontiguousarray(B.transpose((2, 0, 1)))
+    Y = [BnPm @ Ci for Ci in C]
+    return [np.tensordot(Yi, Yi, axes=([1, 2], [1, 2])) for Yi in Y]
+
+
+def build_JK(B, C, BnPm=None):
+    """(J, K) lists for C_k with D_k = C_k C_k^T (jkbuilder.py:209-251, real case)."""
+    D = [np.einsum("mi,ni->mn", Ci, Ci, optimize=True) for Ci in C]
+    return build_J(B, D), build_K_nPm(B, C, BnPm)
+
+
+def build_JK_generalized(B, C, g1):
+    """J/K of the density C g1 C^T via the Cholesky (or eigen) factor of g1 (jkbuilder.py:253-290)."""
+    assert C.shape[1] == g1.shape[0]
+    try:
+        L = np.linalg.cholesky(g1)
+        Cp = C @ L
+    except np.linalg.LinAlgError:
+        n, L = np.linalg.eigh(g1)
+        assert np.all(n > -1.0e-11), "g1 must be positive semi-definite"
+        n = np.maximum(n, 0)
+        Cp = C @ L @ np.diag(np.sqrt(n))
+    J, K = build_JK(B, [Cp])
+    return J[0], K[0]
+
+
+def build_core_fock(B, C_core, hcore):
+    """h + 2 J - K of the core orbitals (jkbuilder.py:292-321)."""
+    J, K = build_JK(B, [C_core])
+    return hcore + 2.0 * J[0] - K[0]
+
+
+def build_active_fock(B, C_act, g1):
+    """J - K/2 of the active density (jkbuilder.py:323-349)."""
+    J, K = build_JK_generalized(B, C_act, g1)
+    return J - 0.5 * K
+
+
+def eri_from_B(B):
+    """(mn|rs) = sum_P B_Pmn B_Prs -- brute-force helper for small N."""
+    return np.einsum("Pmn,Prs->mnrs", B, B)

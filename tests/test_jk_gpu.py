@@ -1,0 +1,121 @@
+"""GPU parity of the HIP J/K kernels against the numpy oracle (oracle/jk.py).
+
+Plain form: FP64 J and K must match the oracle to <= 1e-10 max-abs (the bar of
+the plain FP64 form).  Shapes cover the water and (H2O)_16 workloads, odd nbf
+(scalar loads), nbf not a multiple of the 128 tile, nocc not a multiple of 16,
+several densities per call (UHF), column-slice C with a row stride, and the
+FP32 precision form (relative tolerance stated per test).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from forte2_amd.jkbuilder import FockBuilder
+from forte2_amd.workloads import orthonormal_C, synthetic_B
+from oracle import jk as ojk
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+
+
+def _case(dev, Q, N, noccs, seed):
+    print(f"[jk-gpu] Q={Q} N={N} nocc={noccs} seed={seed}", flush=True)
+    B = synthetic_B(Q, N, seed=seed, device=dev)
+    Cs = [orthonormal_C(N, o, seed=seed + i, device=dev) for i, o in enumerate(noccs)]
+    Bh = B.cpu().numpy()
+    Ch = [c.cpu().numpy() for c in Cs]
+    return B, Cs, Bh, Ch
+
+
+@pytest.mark.parametrize(
+    "Q,N,noccs",
+    [
+        (139, 24, [5]),           # water
+        (100, 19, [5]),           # odd nbf: scalar J loads, scalar KX loads
+        (300, 130, [17, 13]),     # two densities, nbf just over one tile
+        (50, 64, [7, 3, 20]),     # three densities (J processed 2 + 1)
+        (2224, 384, [80]),        # (H2O)_16
+        (2224, 384, [80, 79]),    # (H2O)_16, UHF-like
+    ],
+)
+def test_build_jk_fp64_matches_oracle(dev, Q, N, noccs):
+    B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=Q + N)
+    fb = FockBuilder(B=B, device=dev)
+    J, K = fb.build_JK(Cs)
+    torch.cuda.synchronize()
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    for k in range(len(noccs)):
+        ej = np.abs(J[k].cpu().numpy() - Jr[k]).max()
+        ek = np.abs(K[k].cpu().numpy() - Kr[k]).max()
+        print(f"   dens {k}: |dJ|={ej:.2e} |dK|={ek:.2e} max|J|={np.abs(Jr[k]).max():.2e}", flush=True)
+        assert ej <= TOL_FP64
+        assert ek <= TOL_FP64
+
+
+def test_build_j_and_k_separately(dev):
+    Q, N = 400, 96
+    B, Cs, Bh, Ch = _case(dev, Q, N, [11, 6], seed=5)
+    fb = FockBuilder(B=B, device=dev)
+    D = [c @ c.T for c in Cs]
+    J = fb.build_J(D)
+    K = fb.build_K(Cs)
+    Jr = ojk.build_J(Bh, [c @ c.T for c in Ch])
+    Kr = ojk.build_K_nPm(Bh, Ch)
+    for k in range(2):
+        assert np.abs(J[k].cpu().numpy() - Jr[k]).max() <= TOL_FP64
+        assert np.abs(K[k].cpu().numpy() - Kr[k]).max() <= TOL_FP64
+
+
+def test_numpy_column_slice_inputs(dev):
+    # the reference passes C[:, :nocc] slices of an (nbf, nmo) array (rhf.py:36)
+    Q, N, nmo, na = 139, 24, 24, 5
+    B = synthetic_B(Q, N, seed=3, device=dev)
+    Cfull = orthonormal_C(N, nmo, seed=3, device="cpu").numpy()
+    fb = FockBuilder(B=B.cpu().numpy(), device=dev)
+    J, K = fb.build_JK([Cfull[:, :na]])
+    assert isinstance(J[0], np.ndarray) and isinstance(K[0], np.ndarray)
+    Jr, Kr = ojk.build_JK(B.cpu().numpy(), [Cfull[:, :na]])
+    assert np.abs(J[0] - Jr[0]).max() <= TOL_FP64
+    assert np.abs(K[0] - Kr[0]).max() <= TOL_FP64
+
+
+def test_small_xchunk_and_split_paths(dev):
+    # force several X chunks and a large split count: same result bit for bit
+    Q, N = 700, 200
+    B, Cs, Bh, Ch = _case(dev, Q, N, [33], seed=9)
+    fb = FockBuilder(B=B, device=dev)
+    J0, K0 = fb.build_JK(Cs)
+    fb2 = FockBuilder(B=B, device=dev)
+    fb2.set_tuning("xchunk_bytes", 1 << 20)
+    J1, K1 = fb2.build_JK(Cs)
+    Kr = ojk.build_K_nPm(Bh, Ch)[0]
+    assert np.abs(K1[0].cpu().numpy() - Kr).max() <= TOL_FP64
+    assert torch.equal(J0[0], J1[0])
+    # repeated builds are bitwise reproducible
+    J2, K2 = fb.build_JK(Cs)
+    assert torch.equal(J0[0], J2[0]) and torch.equal(K0[0], K2[0])
+
+
+def test_fp32_precision_form(dev):
+    Q, N = 2224, 384
+    B, Cs, Bh, Ch = _case(dev, Q, N, [80], seed=1)
+    fb = FockBuilder(B=B, device=dev)
+    J, K = fb.build_JK(Cs, precision="fp32")
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    rj = np.abs(J[0].cpu().numpy() - Jr[0]).max() / np.abs(Jr[0]).max()
+    rk = np.abs(K[0].cpu().numpy() - Kr[0]).max() / np.abs(Kr[0]).max()
+    print(f"   fp32: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
+    # FP32 operands (unit roundoff 6e-8) with FP64 accumulation across P
+    assert rj < 1e-6
+    assert rk < 1e-5
+
+
+def test_error_paths(dev):
+    B = synthetic_B(10, 8, seed=0, device=dev)
+    fb = FockBuilder(B=B, device=dev)
+    with pytest.raises(ValueError):
+        fb.build_JK([torch.zeros((9, 2), dtype=torch.float64, device=dev)])
+    with pytest.raises(ValueError):
+        fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)], precision="bf16x3")
