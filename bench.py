@@ -1,0 +1,213 @@
+"""Benchmark of the density-fitted J/K build on one MI355X.
+
+Workload: (H2O)_64 with the cc-pVDZ orbital basis and cc-pVTZ-JKFIT auxiliary
+basis shapes (nbf N=1536, naux Q=8896, nocc o=320; the cc-pVTZ-JKFIT fit set
+stands in for the unshipped cc-pVDZ-JKFIT).  B(P|mn) (168 GB FP64 + 84 GB FP32
+copy) and the occupied coefficients are synthetic and already resident in HBM
+when the timed region starts.  One step = one RHF J/K build (D = C C^T, J, K).
+
+The headline value is the builds/s of the precision form (``--mixed``,
+default fp32); the FP64 plain form is timed on the same inputs, and the line
+reports the speed-up and the max-abs J/K difference between the two.
+
+Printed: one JSON line with metric/value/unit, the roofline of the dominant
+kernel (K half-transform, measured with HIP events on the launch stream) and
+the CPU baseline (the numpy oracle on a bounded aux-slice sample, scaled).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "JK builds/s + MFMA util, (H2O)_64 mixed vs FP64 at 1e-8 Eh; p99 SCF latency"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--waters", type=int, default=64, help="(H2O)_n cluster size")
+    ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--cpu-sample-aux", type=int, default=384,
+                    help="aux functions in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--profile-steps", action="store_true",
+                    help="run only the timed mixed steps (for rocprofv3)")
+    return ap.parse_args()
+
+
+def timed_builds(fb, C, prec, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fb.build_JK([C], precision=prec)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        J, K = fb.build_JK([C], precision=prec)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, J[0], K[0]
+
+
+def kernel_times(fb, C, prec, reps=3):
+    """Per-phase HIP-event times (ms) of one build, averaged over reps."""
+    import torch
+
+    fb.timing = True
+    acc, launches = {}, None
+    for _ in range(reps):
+        fb.build_JK([C], precision=prec)
+        for k, v in fb.last_timings.items():
+            acc[k] = acc.get(k, 0.0) + v / reps
+        launches = fb.last_launches
+    fb.timing = False
+    torch.cuda.synchronize()
+    return acc, launches
+
+
+def cpu_baseline(B, C, q_sample):
+    """The numpy oracle (forte2 FockBuilder.build_JK restated) on an aux slice.
+
+    J and K are sums over the aux index P, so the time of a build scales
+    linearly with naux; builds/s of the full problem = sample builds/s *
+    q_sample / naux.
+    """
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle import jk as ojk
+
+    cores = len(os.sched_getaffinity(0))
+    Bs = B[:q_sample].cpu().numpy()
+    Ch = C.cpu().numpy()
+    with threadpool_limits(limits=cores):
+        BnPm = np.ascontiguousarray(Bs.transpose((2, 0, 1)))
+        ojk.build_JK(Bs, [Ch], BnPm)  # warm-up
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ojk.build_JK(Bs, [Ch], BnPm)
+            ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    Q = B.shape[0]
+    return {
+        "value": (1.0 / t) * q_sample / Q,
+        "unit": "builds/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"oracle build_JK (numpy FP64, B_nPm path) on B[:{q_sample}] of the same "
+                  f"(H2O)_n tensor (naux {Q}), median of 3 after 1 warm-up, "
+                  f"{t:.2f} s/sample-build, scaled by {q_sample}/{Q}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from forte2_amd.jkbuilder import FockBuilder
+    from forte2_amd.workloads import orthonormal_C, synthetic_B, water_cluster_shape
+
+    dev = torch.device("cuda", 0)
+    N, Q, o = water_cluster_shape(args.waters)
+    B = synthetic_B(Q, N, seed=2024, device=dev)
+    C = orthonormal_C(N, o, seed=2024, device=dev)
+    fb = FockBuilder(B=B, device=dev)
+    fb.B_fp32()
+    torch.cuda.synchronize()
+    print(f"[bench] (H2O)_{args.waters}: N={N} Q={Q} o={o} inputs resident", file=sys.stderr, flush=True)
+
+    if args.profile_steps:
+        timed_builds(fb, C, args.mixed, args.steps, args.warmup)
+        return
+
+    t_mix, Jm, Km = timed_builds(fb, C, args.mixed, args.steps, args.warmup)
+    print(f"[bench] {args.mixed}: {t_mix*1e3:.2f} ms/build", file=sys.stderr, flush=True)
+    t64, J64, K64 = timed_builds(fb, C, "fp64", max(2, args.steps // 2), 1)
+    print(f"[bench] fp64: {t64*1e3:.2f} ms/build", file=sys.stderr, flush=True)
+    err_j = float((Jm - J64).abs().max())
+    err_k = float((Km - K64).abs().max())
+
+    ph_mix, la_mix = kernel_times(fb, C, args.mixed)
+    ph64, la64 = kernel_times(fb, C, "fp64")
+    jb_mix, kf = fb.work([o], args.mixed)
+    jb64, _ = fb.work([o], "fp64")
+    kx_flops = 2.0 * Q * N * N * o          # algorithmic half-transform flops of one build
+    kx_ms_launch = ph_mix["kx"] / max(1, la_mix["kx"])
+    kx_flops_launch = kx_flops / max(1, la_mix["kx"])
+    achieved = kx_flops_launch / (kx_ms_launch * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.mixed]
+    k_ms = ph_mix["kx"] + ph_mix["ks"] + ph_mix["k_reduce"]
+    k64_ms = ph64["kx"] + ph64["ks"] + ph64["k_reduce"]
+    j_ms = ph_mix["j_gamma"] + ph_mix["j_acc"]
+    j64_ms = ph64["j_gamma"] + ph64["j_acc"]
+
+    cpu = None
+    if args.cpu_sample_aux > 0:
+        cpu = cpu_baseline(B, C, min(args.cpu_sample_aux, Q))
+        print(f"[bench] cpu baseline: {cpu['value']:.4f} builds/s on {cpu['cores']} cores",
+              file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": 1.0 / t_mix,
+        "unit": "builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mix * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.mixed == "fp32" else "bf16x3/fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"(H2O)_{args.waters} RHF J/K build, cc-pVDZ / cc-pVTZ-JKFIT shapes "
+                        f"(nbf {N}, naux {Q}, nocc {o}), B full-square (P,m,n) in HBM",
+            "precision_form": args.mixed,
+            "nbf": N, "naux": Q, "nocc": o,
+        },
+        "fp64": {"builds_per_s": 1.0 / t64, "ms_per_build": t64 * 1e3},
+        "speedup_vs_fp64": t64 / t_mix,
+        "max_abs_diff_vs_fp64": {"J": err_j, "K": err_k},
+        "phases_ms": {args.mixed: ph_mix, "fp64": ph64},
+        "J": {
+            "GBps": jb_mix / (j_ms * 1e-3) / 1e9, "fp64_GBps": jb64 / (j64_ms * 1e-3) / 1e9,
+            "frac_of_hbm_peak": jb_mix / (j_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "bytes_algorithmic": jb_mix, "layout": "lower triangle read from full-square B",
+        },
+        "K": {
+            "TFLOPs": kf / (k_ms * 1e-3) / 1e12, "mfma_util": kf / (k_ms * 1e-3) / 1e12 / peak,
+            "fp64_TFLOPs": kf / (k64_ms * 1e-3) / 1e12,
+            "fp64_mfma_util": kf / (k64_ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"],
+            "flops_algorithmic": kf, "flops_full_gemm": 4.0 * Q * N * N * o,
+        },
+        "roofline": {
+            "kernel": "kx_kernel (K half-transform X = B C)",
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "launches_per_build": la_mix["kx"],
+            "avg_launch_ms": kx_ms_launch,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

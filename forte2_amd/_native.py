@@ -30,6 +30,7 @@ F2JK_SYMBOLS = (
     "f2jk_build_j",
     "f2jk_build_k",
     "f2jk_last_timings",
+    "f2jk_last_launches",
     "f2jk_work",
     "f2jk_last_error",
 )
@@ -71,6 +72,7 @@ def jk_lib():
     lib.f2jk_build_j.argtypes = [V, I, c_voidp_p, I64, c_voidp_p, I, I, V, sz, V]
     lib.f2jk_build_k.argtypes = [V, I, c_voidp_p, c_int64_p, c_int64_p, c_voidp_p, I, I, V, sz, V]
     lib.f2jk_last_timings.argtypes = [V, c_double_p, I]
+    lib.f2jk_last_launches.argtypes = [V, ctypes.POINTER(I), I]
     lib.f2jk_work.argtypes = [V, I, c_int64_p, I, I, c_double_p, c_double_p]
     lib.f2jk_last_error.argtypes = []
     lib.f2jk_last_error.restype = ctypes.c_char_p

@@ -577,6 +577,15 @@ int f2jk_last_timings(f2jk_ctx* ctx, double* ms, int n) {
     return F2JK_OK;
 }
 
+int f2jk_last_launches(const f2jk_ctx* ctx, int* counts, int n) {
+    if (!ctx || !counts) return fail(F2JK_EINVAL, "null argument");
+    int c[F2JK_NPHASE] = {0};
+    for (int i = 0; i + 1 < ctx->nev; i += 2) c[ctx->ev_phase[i / 2]]++;
+    c[F2JK_PH_TOTAL] = ctx->total_recorded ? 1 : 0;
+    for (int i = 0; i < n && i < F2JK_NPHASE; ++i) counts[i] = c[i];
+    return F2JK_OK;
+}
+
 int f2jk_work(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
               double* j_bytes, double* k_flops) {
     if (!ctx) return fail(F2JK_EINVAL, "null context");

@@ -60,6 +60,7 @@ class FockBuilder:
         self._ws = None
         self.timing = False
         self.last_timings = None
+        self.last_launches = None
         if precision not in nat.PREC:
             raise ValueError(f"unknown precision {precision!r}")
         self.precision = precision
@@ -166,6 +167,9 @@ class FockBuilder:
             ms = (ctypes.c_double * len(nat.PHASES))()
             nat.jk_check(nat.jk_lib().f2jk_last_timings(self.ctx, ms, len(nat.PHASES)), "timings")
             self.last_timings = dict(zip(nat.PHASES, list(ms)))
+            cnt = (ctypes.c_int * len(nat.PHASES))()
+            nat.jk_check(nat.jk_lib().f2jk_last_launches(self.ctx, cnt, len(nat.PHASES)), "launches")
+            self.last_launches = dict(zip(nat.PHASES, list(cnt)))
 
     # ------------------------------------------------------------------ builds
     def _jk_call(self, C, want_j, want_k, precision=None):

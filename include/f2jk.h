@@ -121,6 +121,11 @@ int f2jk_build_k(f2jk_ctx* ctx, int ndens, const double* const* C,
  * that build's events.  n <= F2JK_NPHASE values are written. */
 int f2jk_last_timings(f2jk_ctx* ctx, double* ms, int n);
 
+/* Number of timed (start, end) intervals per phase in the last build issued
+ * with F2JK_TIMING (one per kernel launch group: e.g. one KX interval per X
+ * chunk), so that an average launch duration is ms[phase] / counts[phase]. */
+int f2jk_last_launches(const f2jk_ctx* ctx, int* counts, int n);
+
 /* Algorithmic work of one build with these arguments (for roofline
  * reporting): bytes of B streamed by the J passes and K flops
  * (2*Q*N^2*o half-transform + N(N+1)*Q*o symmetric contraction). */
