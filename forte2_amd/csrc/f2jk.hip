@@ -83,8 +83,10 @@ struct Plan {
     int jvec = 1, nvec = 0, neg = 0, npc1 = 0, ppw2 = 0, npc2 = 0;
     size_t jb = 8;             // bytes of a B element read by J
     // K
-    int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1;
+    int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0;
     // workspace offsets
+    int KP = 0;                // nbf rounded up to the bf16 k step
+    size_t off_cp32 = 0, off_cth = 0, off_ctl = 0;
     size_t off_cp = 0, off_d = 0, off_dpart = 0, off_gpart = 0, off_gamma = 0, off_jpart = 0,
            off_x = 0, off_kpart = 0, total = 0;
 };
@@ -119,8 +121,6 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         return fail(F2JK_EINVAL, "ndens must be in [1, " + std::to_string(kMaxDens) + "]");
     if (precision < F2JK_PREC_FP64 || precision > F2JK_PREC_BF16X3)
         return fail(F2JK_EINVAL, "unknown precision");
-    if (precision == F2JK_PREC_BF16X3)
-        return fail(F2JK_EUNSUPPORTED, "BF16X3 precision is not available in this build");
     p.N = (int)ctx->nbf;
     p.Q = (int)ctx->naux;
     p.nd = ndens;
@@ -141,9 +141,10 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
     if (p.need_c) {
         if (!nocc) return fail(F2JK_EINVAL, "nocc is required");
         int o = 0;
+        const int kalign = (precision == F2JK_PREC_BF16X3) ? kBK16 : 16;
         for (int d = 0; d < ndens; ++d) {
             if (nocc[d] < 0 || nocc[d] > N) return fail(F2JK_EINVAL, "nocc out of range");
-            p.width[d] = (int)(cdiv(std::max<int64_t>(nocc[d], 1), 16) * 16);
+            p.width[d] = (int)(cdiv(std::max<int64_t>(nocc[d], 1), kalign) * kalign);
             p.ioff[d] = o;
             o += p.width[d];
         }
@@ -151,7 +152,15 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         // the C copy used for D (always fp64) and the K copy (fp64 or fp32)
         p.off_cp = off;
         off += align_up((size_t)N * p.OPAD * 8);
-        off += align_up((size_t)N * p.OPAD * 4);  // fp32 copy (only used by FP32 K)
+        p.off_cp32 = off;
+        off += align_up((size_t)N * p.OPAD * 4);  // fp32 copy (FP32 K)
+        p.KP = (int)(cdiv(N, kBK16) * kBK16);
+        if (precision == F2JK_PREC_BF16X3) {
+            p.off_cth = off;                       // transposed hi/lo bf16 copies (BF16X3 K)
+            off += align_up((size_t)p.OPAD * p.KP * 2);
+            p.off_ctl = off;
+            off += align_up((size_t)p.OPAD * p.KP * 2);
+        }
     }
     if (p.want_j) {
         p.jb = (jdtype == F2JK_F64) ? 8 : 4;
@@ -186,7 +195,13 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         p.ntile = (int)cdiv(N, kTile);
         p.nlower = p.ntile * (p.ntile + 1) / 2;
         p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(ctx->target_blocks / 2, p.nlower)));
-        p.ncolt = (int)cdiv(p.OPAD, kTile);
+        if (precision == F2JK_PREC_BF16X3) {
+            const int need = (int)cdiv(std::min(p.OPAD, 320), 32);
+            p.nbw = need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : 10;
+            p.ncolt = (int)cdiv(p.OPAD, 32 * p.nbw);
+        } else {
+            p.ncolt = (int)cdiv(p.OPAD, kTile);
+        }
         if (p.tk == 8)
             p.kva = (N % 2 == 0) ? 2 : 1;
         else
@@ -295,7 +310,7 @@ int run_j(f2jk_ctx* ctx, const Plan& p, const double* const* D, int64_t ldd, dou
 int pack_c(f2jk_ctx* ctx, const Plan& p, const double* const* C, const int64_t* nocc,
            const int64_t* ldc, char* ws, hipStream_t st, bool want_f32) {
     double* cp = (double*)(ws + p.off_cp);
-    float* cp32 = (float*)(ws + p.off_cp + align_up((size_t)p.N * p.OPAD * 8));
+    float* cp32 = (float*)(ws + p.off_cp32);
     for (int d = 0; d < p.nd; ++d) {
         if (nocc[d] > 0 && (!C[d] || !ldc || ldc[d] < nocc[d]))
             return fail(F2JK_EINVAL, "C pointer / ldc invalid");
@@ -309,6 +324,13 @@ int pack_c(f2jk_ctx* ctx, const Plan& p, const double* const* C, const int64_t* 
                                C[d], nocc[d] > 0 ? ldc[d] : 1, p.N, (int)nocc[d], p.width[d],
                                p.ioff[d], p.OPAD, cp32);
             F2JK_LAUNCHED("pad_c32");
+        }
+        if (p.want_k && p.prec == F2JK_PREC_BF16X3) {
+            const int64_t nt = (int64_t)p.width[d] * p.KP;
+            hipLaunchKernelGGL(pad_ct_bf16_kernel, dim3((unsigned)cdiv(nt, 256)), dim3(256), 0, st,
+                               C[d], nocc[d] > 0 ? ldc[d] : 1, p.N, (int)nocc[d], p.width[d],
+                               p.ioff[d], p.KP, (u16*)(ws + p.off_cth), (u16*)(ws + p.off_ctl));
+            F2JK_LAUNCHED("pad_ct_bf16");
         }
     }
     return F2JK_OK;
@@ -336,8 +358,7 @@ int build_d(f2jk_ctx* ctx, const Plan& p, char* ws, hipStream_t st) {
 template <typename T>
 int run_k_typed(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStream_t st) {
     const T* B = (const T*)ctx->B[sizeof(T) == 8 ? F2JK_F64 : F2JK_F32];
-    const T* cp = (sizeof(T) == 8) ? (const T*)(ws + p.off_cp)
-                                   : (const T*)(ws + p.off_cp + align_up((size_t)p.N * p.OPAD * 8));
+    const T* cp = (sizeof(T) == 8) ? (const T*)(ws + p.off_cp) : (const T*)(ws + p.off_cp32);
     T* X = (T*)(ws + p.off_x);
     double* kpart = (double*)(ws + p.off_kpart);
     const int64_t NN = (int64_t)p.N * p.N;
@@ -367,6 +388,66 @@ int run_k_typed(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStr
                                p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
                                kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
             F2JK_LAUNCHED("ks");
+        }
+        TIM(F2JK_PH_KS, false);
+    }
+    TIM(F2JK_PH_KRED, true);
+    const int nsplit_used = std::min(p.nsplit, std::min(p.Pc, p.Q));
+    for (int d = 0; d < p.nd; ++d) {
+        hipLaunchKernelGGL(k_reduce_kernel, dim3((unsigned)cdiv(p.N, 256), p.N), dim3(256), 0, st,
+                           kpart + (size_t)d * p.nsplit * NN, nsplit_used, p.N, K[d]);
+        F2JK_LAUNCHED("k_reduce");
+    }
+    TIM(F2JK_PH_KRED, false);
+    return F2JK_OK;
+}
+
+template <int NBW>
+void launch_kx_bf16(const Plan& p, unsigned grid, const float* A, int64_t R, const u16* th,
+                    const u16* tl, u16* xh, u16* xl, hipStream_t st) {
+    if (p.kva == 4)
+        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 4>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N,
+                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+    else if (p.kva == 2)
+        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 2>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N,
+                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+    else
+        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 1>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N,
+                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+}
+
+int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStream_t st) {
+    const float* B = (const float*)ctx->B[F2JK_F32];
+    const u16* th = (const u16*)(ws + p.off_cth);
+    const u16* tl = (const u16*)(ws + p.off_ctl);
+    // X chunk: hi half then lo half
+    u16* xh = (u16*)(ws + p.off_x);
+    u16* xl = xh + (size_t)p.Pc * p.N * p.OPAD;
+    double* kpart = (double*)(ws + p.off_kpart);
+    const int64_t NN = (int64_t)p.N * p.N;
+    for (int c = 0; c < p.nchunks; ++c) {
+        const int P0 = c * p.Pc;
+        const int pc = std::min(p.Pc, p.Q - P0);
+        const int64_t R = (int64_t)pc * p.N;
+        const unsigned grid = (unsigned)(cdiv(R, kTile) * p.ncolt);
+        const float* A = B + (int64_t)P0 * NN;
+        TIM(F2JK_PH_KX, true);
+        switch (p.nbw) {
+            case 2: launch_kx_bf16<2>(p, grid, A, R, th, tl, xh, xl, st); break;
+            case 4: launch_kx_bf16<4>(p, grid, A, R, th, tl, xh, xl, st); break;
+            case 6: launch_kx_bf16<6>(p, grid, A, R, th, tl, xh, xl, st); break;
+            case 8: launch_kx_bf16<8>(p, grid, A, R, th, tl, xh, xl, st); break;
+            default: launch_kx_bf16<10>(p, grid, A, R, th, tl, xh, xl, st); break;
+        }
+        F2JK_LAUNCHED("kx_bf16x3");
+        TIM(F2JK_PH_KX, false);
+        TIM(F2JK_PH_KS, true);
+        const int nsplit = std::min(p.nsplit, pc);
+        for (int d = 0; d < p.nd; ++d) {
+            hipLaunchKernelGGL(ks_bf16x3_kernel, dim3(p.nlower * nsplit), dim3(kGThreads), 0, st, xh, xl,
+                               p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
+                               kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
+            F2JK_LAUNCHED("ks_bf16x3");
         }
         TIM(F2JK_PH_KS, false);
     }
@@ -411,7 +492,7 @@ int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t
     TIM(F2JK_PH_TOTAL, true);
     if (p.need_c) {
         TIM(F2JK_PH_PREP, true);
-        if ((rc = pack_c(ctx, p, C, nocc, ldc, w, st, p.want_k && p.tk == 4))) return rc;
+        if ((rc = pack_c(ctx, p, C, nocc, ldc, w, st, p.want_k && p.prec == F2JK_PREC_FP32))) return rc;
         if (p.want_j && j_from_c)
             if ((rc = build_d(ctx, p, w, st))) return rc;
         TIM(F2JK_PH_PREP, false);
@@ -425,7 +506,10 @@ int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t
         if ((rc = run_j(ctx, p, dptr.data(), lddu, J, w, st))) return rc;
     }
     if (p.want_k) {
-        rc = (p.tk == 8) ? run_k_typed<double>(ctx, p, K, w, st) : run_k_typed<float>(ctx, p, K, w, st);
+        if (p.prec == F2JK_PREC_BF16X3)
+            rc = run_k_bf16x3(ctx, p, K, w, st);
+        else
+            rc = (p.tk == 8) ? run_k_typed<double>(ctx, p, K, w, st) : run_k_typed<float>(ctx, p, K, w, st);
         if (rc) return rc;
     }
     TIM(F2JK_PH_TOTAL, false);

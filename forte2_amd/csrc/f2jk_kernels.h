@@ -366,7 +366,8 @@ __global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, 
     __shared__ T As[kTile][kBK + 1];
     __shared__ T Bs[kBK][kTile];
 
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int64_t r0 = (int64_t)(t / ncolt) * kTile;
     const int c0 = (t % ncolt) * kTile;
@@ -482,7 +483,8 @@ __global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, 
     __shared__ T As[kTile][kBK + 1];
     __shared__ T Bs[kTile][kBK + 1];
 
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int lt = xcd_remap(blockIdx.x, gridDim.x);
     const int s = lt / nlower;
     int tl = lt % nlower;
@@ -596,6 +598,316 @@ __global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, 
                 double* dst = Ks + (int64_t)row * N + col;
                 const double v = outval(i, j, r);
                 *dst = accumulate ? (*dst + v) : v;
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 precision form.  An FP32 value x is carried as the bf16 pair
+//   hi = rne_bf16(x),  lo = rne_bf16(x - hi)      (x - hi is exact in FP32)
+// which keeps ~16 significant bits; a product is formed from three bf16 MFMAs
+//   a*b ~ hi_a*hi_b + hi_a*lo_b + lo_a*hi_b     (lo_a*lo_b ~ 2^-32 relative, dropped)
+// with FP32 accumulation.  v_mfma_f32_16x16x32_bf16: lane l supplies
+// A[l%16][8*(l/16) + 0..7] and B[8*(l/16) + 0..7][l%16]; accumulator element
+// r holds D[4*(l/16) + r][l%16].
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int kBK16 = 32;              // k extent of one LDS stage = one MFMA k step
+constexpr int kPitch16 = kBK16 + 8;    // LDS row pitch (bf16): 80 B rows, conflict-free 16 B reads
+
+__device__ __forceinline__ fx4 mfma_bf16(bf16x8 a, bf16x8 b, fx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// CpT[c][k] (hi/lo) = Cp[k][c] for c < OPAD, k < N; zero for N <= k < KP.
+__global__ void pad_ct_bf16_kernel(const double* __restrict__ C, int64_t ldc, int N, int nocc,
+                                   int width, int ioff, int KP, u16* __restrict__ Th,
+                                   u16* __restrict__ Tl) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)width * KP) return;
+    const int i = (int)(t / KP), k = (int)(t % KP);
+    const float x = (i < nocc && k < N) ? (float)C[(int64_t)k * ldc + i] : 0.f;
+    __bf16 h, l;
+    split_bf16(x, h, l);
+    const int64_t o = (int64_t)(ioff + i) * KP + k;
+    Th[o] = __builtin_bit_cast(u16, h);
+    Tl[o] = __builtin_bit_cast(u16, l);
+}
+
+// KX (bf16x3): X[r][c] = sum_k A[r][k] Ct[c][k]
+//   A  = FP32 B chunk, R x K row-major, split into hi/lo while staged to LDS
+//   Ct = pre-split transposed coefficients, OPAD x KP (KP = K rounded up to 32)
+//   X  = hi/lo bf16 pairs, R x OPAD.
+// Tile 128 x (32*NBW): one block holds up to 320 occupied columns so that B is
+// read from HBM once.  4 waves as 2 x 2, each 64 x (16*NBW).
+template <int NBW, int VA>
+__global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
+    const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
+    const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt) {
+    constexpr int BN = 32 * NBW;
+    __shared__ __bf16 Ah[kTile][kPitch16];
+    __shared__ __bf16 Al[kTile][kPitch16];
+    __shared__ __bf16 Bh[BN][kPitch16];
+    __shared__ __bf16 Bl[BN][kPitch16];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t r0 = (int64_t)(t / ncolt) * kTile;
+    const int c0 = (t % ncolt) * BN;
+    const int wr = wid >> 1, wc = wid & 1;
+    const int wcol0 = c0 + wc * 16 * NBW;
+    int nbv = (OPAD - wcol0) / 16;
+    nbv = nbv < 0 ? 0 : (nbv > NBW ? NBW : nbv);
+
+    constexpr int A_LD = (kTile * kBK16) / (kGThreads * VA);
+    constexpr int B_LD = (BN * kBK16 / 8) / kGThreads;   // 16-byte loads per array
+    static_assert((BN * kBK16 / 8) % kGThreads == 0, "B tile must split evenly");
+    Vec<float, VA> ra[A_LD];
+    uint4 rbh[B_LD], rbl[B_LD];
+
+    auto load_tiles = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
+            const int64_t gr = r0 + row;
+            const int gk = k0 + kc;
+            if (gr < R && gk < K) {
+                ra[s] = load_vec<float, VA>(A + gr * K + gk);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VA; ++v) ra[s].v[v] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < B_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
+            const int gc = c0 + row;
+            if (gc < OPAD) {
+                rbh[s] = *reinterpret_cast<const uint4*>(Cth + (int64_t)gc * KP + k0 + kc);
+                rbl[s] = *reinterpret_cast<const uint4*>(Ctl + (int64_t)gc * KP + k0 + kc);
+            } else {
+                rbh[s] = make_uint4(0, 0, 0, 0);
+                rbl[s] = make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto store_tiles = [&]() {
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
+#pragma unroll
+            for (int v = 0; v < VA; ++v) split_bf16(ra[s].v[v], Ah[row][kc + v], Al[row][kc + v]);
+        }
+#pragma unroll
+        for (int s = 0; s < B_LD; ++s) {
+            const int idx = tid + kGThreads * s;
+            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
+            *reinterpret_cast<uint4*>(&Bh[row][kc]) = rbh[s];
+            *reinterpret_cast<uint4*>(&Bl[row][kc]) = rbl[s];
+        }
+    };
+
+    fx4 acc[4][NBW];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) acc[i][j] = fx4{0, 0, 0, 0};
+
+    const int nk = KP / kBK16;
+    load_tiles(0);
+    for (int kt = 0; kt < nk; ++kt) {
+        store_tiles();
+        __syncthreads();
+        if (kt + 1 < nk) load_tiles((kt + 1) * kBK16);
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+        }
+        // the three products of a block are issued in separate sweeps so that
+        // consecutive MFMAs never depend on each other
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) {
+            if (j < nbv) {
+                const int br = wc * 16 * NBW + j * 16 + (lane & 15);
+                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bh[br][8 * (lane >> 4)]);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bl[br][8 * (lane >> 4)]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NBW; ++j) {
+            if (j >= nbv) continue;
+            const int col = wcol0 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+                if (row < R) {
+                    __bf16 h, l;
+                    split_bf16(acc[i][j][r], h, l);
+                    Xh[row * OPAD + col] = __builtin_bit_cast(u16, h);
+                    Xl[row * OPAD + col] = __builtin_bit_cast(u16, l);
+                }
+            }
+        }
+}
+
+// KS (bf16x3): Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
+// with X given as hi/lo bf16 pairs; width % 32 == 0.  FP32 accumulation over
+// one P, folded into FP64 registers after every P.
+__global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
+    const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
+    int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate) {
+    __shared__ __bf16 Ah[kTile][kPitch16];
+    __shared__ __bf16 Al[kTile][kPitch16];
+    __shared__ __bf16 Bh[kTile][kPitch16];
+    __shared__ __bf16 Bl[kTile][kPitch16];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    const int lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int s = lt / nlower;
+    const int tl = lt % nlower;
+    int mt = 0;
+    while ((mt + 1) * (mt + 2) / 2 <= tl) ++mt;
+    const int nt = tl - mt * (mt + 1) / 2;
+    const int m0 = mt * kTile, n0 = nt * kTile;
+    const int wr = wid >> 1, wc = wid & 1;
+    const int Pa = (int)(((int64_t)Pc * s) / nsplit);
+    const int Pb = (int)(((int64_t)Pc * (s + 1)) / nsplit);
+
+    constexpr int LD = (kTile * kBK16 / 8) / kGThreads;  // 16-byte loads per array (= 2)
+    uint4 rah[LD], ral[LD], rbh[LD], rbl[LD];
+
+    auto load_tiles = [&](int P, int i0) {
+        const int64_t base = (int64_t)P * N * OPAD + ioff + i0;
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            const int idx = tid + kGThreads * q;
+            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
+            if (m0 + row < N) {
+                const int64_t o = base + (int64_t)(m0 + row) * OPAD + kc;
+                rah[q] = *reinterpret_cast<const uint4*>(Xh + o);
+                ral[q] = *reinterpret_cast<const uint4*>(Xl + o);
+            } else {
+                rah[q] = make_uint4(0, 0, 0, 0);
+                ral[q] = make_uint4(0, 0, 0, 0);
+            }
+            if (n0 + row < N) {
+                const int64_t o = base + (int64_t)(n0 + row) * OPAD + kc;
+                rbh[q] = *reinterpret_cast<const uint4*>(Xh + o);
+                rbl[q] = *reinterpret_cast<const uint4*>(Xl + o);
+            } else {
+                rbh[q] = make_uint4(0, 0, 0, 0);
+                rbl[q] = make_uint4(0, 0, 0, 0);
+            }
+        }
+    };
+    auto store_tiles = [&]() {
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            const int idx = tid + kGThreads * q;
+            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
+            *reinterpret_cast<uint4*>(&Ah[row][kc]) = rah[q];
+            *reinterpret_cast<uint4*>(&Al[row][kc]) = ral[q];
+            *reinterpret_cast<uint4*>(&Bh[row][kc]) = rbh[q];
+            *reinterpret_cast<uint4*>(&Bl[row][kc]) = rbl[q];
+        }
+    };
+
+    dx4 acc64[4][4];
+    fx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc64[i][j] = dx4{0, 0, 0, 0};
+            acc[i][j] = fx4{0, 0, 0, 0};
+        }
+
+    const int nkp = width / kBK16;
+    const int nsteps = (Pb - Pa) * nkp;
+    if (nsteps > 0) load_tiles(Pa, 0);
+    for (int st = 0; st < nsteps; ++st) {
+        store_tiles();
+        __syncthreads();
+        if (st + 1 < nsteps) {
+            const int sn = st + 1;
+            load_tiles(Pa + sn / nkp, (sn % nkp) * kBK16);
+        }
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+        }
+        bf16x8 bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int br = wc * 64 + j * 16 + (lane & 15);
+            bh[j] = *reinterpret_cast<const bf16x8*>(&Bh[br][8 * (lane >> 4)]);
+            bl[j] = *reinterpret_cast<const bf16x8*>(&Bl[br][8 * (lane >> 4)]);
+        }
+        // lo*hi, hi*lo, then hi*hi: each sweep touches all 16 accumulators once
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+        __syncthreads();
+        if ((st + 1) % nkp == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
+                    acc[i][j] = fx4{0, 0, 0, 0};
+                }
+        }
+    }
+    double* Ks = Kpart + (int64_t)s * N * N;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+                if (row >= N) continue;
+                double* dst = Ks + (int64_t)row * N + col;
+                *dst = accumulate ? (*dst + acc64[i][j][r]) : acc64[i][j][r];
             }
         }
 }

@@ -112,10 +112,36 @@ def test_fp32_precision_form(dev):
     assert rk < 1e-5
 
 
+@pytest.mark.parametrize(
+    "Q,N,noccs",
+    [
+        (2224, 384, [80]),        # (H2O)_16
+        (300, 130, [17, 13]),     # two densities, nbf over one tile, nocc not a multiple of 32
+        (100, 19, [5]),           # odd nbf: scalar FP32 loads in the split path
+        (500, 256, [330]),        # more than 320 occupied columns: two column tiles
+    ],
+)
+def test_bf16x3_precision_form(dev, Q, N, noccs):
+    B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=17)
+    fb = FockBuilder(B=B, device=dev)
+    J, K = fb.build_JK(Cs, precision="bf16x3")
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    for k in range(len(noccs)):
+        rj = np.abs(J[k].cpu().numpy() - Jr[k]).max() / np.abs(Jr[k]).max()
+        rk = np.abs(K[k].cpu().numpy() - Kr[k]).max() / np.abs(Kr[k]).max()
+        print(f"   bf16x3 dens {k}: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
+        # J streams the FP32 copy of B (FP64 accumulation); K carries ~16
+        # significant bits per operand (hi/lo bf16 pair) with FP32 accumulation
+        assert rj < 1e-6
+        assert rk < 1e-4
+
+
 def test_error_paths(dev):
     B = synthetic_B(10, 8, seed=0, device=dev)
     fb = FockBuilder(B=B, device=dev)
     with pytest.raises(ValueError):
         fb.build_JK([torch.zeros((9, 2), dtype=torch.float64, device=dev)])
     with pytest.raises(ValueError):
-        fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)], precision="bf16x3")
+        fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)] * 9)
+    with pytest.raises(ValueError):
+        fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)], precision="fp16")
