@@ -1,0 +1,769 @@
+// Host integral engine: McMurchie-Davidson Hermite expansions, solid-harmonic
+// shells, OpenMP over shell pairs.  ABI in include/f2ints.h.
+//
+// Method (T. Helgaker, P. Jorgensen, J. Olsen, "Molecular Electronic-Structure
+// Theory", ch. 9):
+//   * a Cartesian Gaussian pair x_A^i x_B^j e^{-a x_A^2 - b x_B^2} is expanded in
+//     Hermite Gaussians Lambda_t(p, P) with coefficients E^{ij}_t (eq. 9.5.6-7);
+//   * Coulomb-type integrals over Hermite Gaussians reduce to the auxiliary
+//     integrals R_{tuv}(alpha, R_PQ) built from the Boys function (9.9.18-20);
+//   * a single-centre solid-harmonic Gaussian S_lm(r_C) e^{-g r_C^2} equals
+//     (2g)^{-l} S_lm(d/dC) e^{-g r_C^2} exactly (Hobson's theorem, S_lm is
+//     harmonic), so auxiliary shells contribute only order-l Hermite terms;
+//   * the Boys function F_n(T) is tabulated on a grid (series, then downward
+//     recursion), evaluated by a 7-term Taylor expansion at the highest order
+//     and downward recursion below it; for T > 36 the asymptotic F_0 and
+//     upward recursion are exact to double precision (9.8.2).
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "f2ints.h"
+
+namespace {
+
+constexpr int LMAX = 6;                  // up to i shells
+constexpr int NBOYS = 4 * LMAX + 1;      // highest Boys order needed by (ab|cd)
+constexpr double PI = 3.14159265358979323846;
+constexpr double kPrimScreen = 1.0e-15;  // primitive pairs below this prefactor are skipped
+
+thread_local std::string g_err;
+int g_threads = 0;
+
+int fail(const std::string& m) {
+    g_err = m;
+    return -1;
+}
+
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+inline int nsph(int l) { return 2 * l + 1; }
+inline int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
+// ---------------------------------------------------------------- Boys function
+struct BoysTable {
+    static constexpr double h = 0.05;
+    static constexpr double Tswitch = 36.0;
+    static constexpr int NT = NBOYS + 8;  // orders kept per grid point
+    int npts;
+    std::vector<double> tab;
+    BoysTable() {
+        npts = (int)(Tswitch / h) + 3;
+        tab.resize((size_t)npts * NT);
+        for (int i = 0; i < npts; ++i) {
+            const long double T = (long double)i * h;
+            // F_n(T) = e^{-T} sum_k (2T)^k / ((2n+1)(2n+3)...(2n+2k+1))
+            const int n = NT - 1;
+            long double term = 1.0L / (2 * n + 1), sum = term;
+            for (int k = 1; k < 2000; ++k) {
+                term *= 2.0L * T / (2 * n + 2 * k + 1);
+                sum += term;
+                if (term < 1e-22L * sum) break;
+            }
+            const long double eT = expl(-T);
+            long double F = eT * sum;
+            tab[(size_t)i * NT + n] = (double)F;
+            for (int m = n - 1; m >= 0; --m) {
+                F = (2.0L * T * F + eT) / (2 * m + 1);
+                tab[(size_t)i * NT + m] = (double)F;
+            }
+        }
+    }
+    void eval(int nmax, double T, double* F) const {
+        if (T < Tswitch) {
+            const int i = (int)(T / h + 0.5);
+            const double d = T - i * h;
+            const double* t = &tab[(size_t)i * NT];
+            // F_nmax(T) = sum_k F_{nmax+k}(T_i) (-d)^k / k!
+            double f = 0.0, c = 1.0;
+            for (int k = 0; k <= 6; ++k) {
+                f += t[nmax + k] * c;
+                c *= -d / (k + 1);
+            }
+            F[nmax] = f;
+            const double eT = std::exp(-T);
+            for (int m = nmax - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + eT) / (2 * m + 1);
+        } else {
+            const double eT = std::exp(-T);
+            F[0] = 0.5 * std::sqrt(PI / T);
+            for (int m = 0; m < nmax; ++m) F[m + 1] = ((2 * m + 1) * F[m] - eT) / (2.0 * T);
+        }
+    }
+};
+const BoysTable& boys() {
+    static BoysTable t;
+    return t;
+}
+
+// ---------------------------------------------------------------- Cartesian / solid harmonics
+struct CartComp {
+    int x, y, z;
+};
+struct SphTerm {
+    int cart;
+    double c;
+};
+struct Harmonics {
+    std::vector<std::vector<CartComp>> cart;                 // [l][ncart]
+    std::vector<std::vector<double>> T;                      // [l][(2l+1) * ncart]
+    std::vector<std::vector<std::vector<SphTerm>>> sparse;   // [l][m] -> terms
+    Harmonics() {
+        cart.resize(LMAX + 1);
+        T.resize(LMAX + 1);
+        sparse.resize(LMAX + 1);
+        for (int l = 0; l <= LMAX; ++l) {
+            for (int a = l; a >= 0; --a)
+                for (int b = l - a; b >= 0; --b) cart[l].push_back({a, b, l - a - b});
+            T[l].assign((size_t)nsph(l) * ncart(l), 0.0);
+            for (int m = -l; m <= l; ++m) build(l, m);
+            sparse[l].resize(nsph(l));
+            for (int s = 0; s < nsph(l); ++s)
+                for (int c = 0; c < ncart(l); ++c) {
+                    const double v = T[l][(size_t)s * ncart(l) + c];
+                    if (v != 0.0) sparse[l][s].push_back({c, v});
+                }
+        }
+    }
+    static double fact(int n) {
+        double f = 1;
+        for (int i = 2; i <= n; ++i) f *= i;
+        return f;
+    }
+    static double binom(double n, double k) {
+        if (k < 0 || k > n) return 0.0;
+        // integer arguments only (k may be passed as an exact integer double)
+        return fact((int)n) / (fact((int)k) * fact((int)(n - k)));
+    }
+    int cart_index(int l, int x, int y, int z) const {
+        for (int c = 0; c < ncart(l); ++c)
+            if (cart[l][c].x == x && cart[l][c].y == y && cart[l][c].z == z) return c;
+        return -1;
+    }
+    // real solid harmonic S_lm (Helgaker eq. 6.4.47-6.4.50), phases and norms of
+    // forte2/integrals/real_spherical_harmonics.hpp (same norm as z^l)
+    void build(int l, int m) {
+        const int am = std::abs(m);
+        const double N = 1.0 / (std::pow(2.0, am) * fact(l)) *
+                         std::sqrt(2.0 * fact(l + am) * fact(l - am) / (m == 0 ? 2.0 : 1.0));
+        const int half = (m < 0) ? 1 : 0;  // v = half/2 + k
+        for (int t = 0; t <= (l - am) / 2; ++t)
+            for (int u = 0; u <= t; ++u) {
+                const int kmax = (m < 0) ? (am - 1) / 2 : am / 2;
+                for (int k = 0; k <= kmax; ++k) {
+                    const int twov = 2 * k + half;  // 2v
+                    const double sign = ((t + k) % 2) ? -1.0 : 1.0;
+                    const double C = sign * std::pow(0.25, t) * binom(l, t) * binom(l - t, am + t) *
+                                     binom(t, u) * binom(am, twov);
+                    if (C == 0.0) continue;
+                    // x^{2t+|m|-2(u+v)} y^{2(u+v)} z^{l-2t-|m|}
+                    const int py = 2 * u + twov;
+                    const int px = 2 * t + am - py;
+                    const int pz = l - 2 * t - am;
+                    const int c = cart_index(l, px, py, pz);
+                    T[l][(size_t)(m + l) * ncart(l) + c] += N * C;
+                }
+            }
+    }
+};
+const Harmonics& harm() {
+    static Harmonics h;
+    return h;
+}
+
+// ---------------------------------------------------------------- Hermite index tables
+struct HermIndex {
+    // idx[L][t][u][v] dense (-1 where t+u+v > L) and lists of (t,u,v) per L
+    std::vector<std::vector<std::array<int, 3>>> list;  // list[L]
+    int LM;
+    std::vector<int> dense;  // [(t*(LM+1)+u)*(LM+1)+v] -> position in list[LM] (order is nested)
+    HermIndex() {
+        LM = 4 * LMAX + 2;
+        list.resize(LM + 1);
+        for (int L = 0; L <= LM; ++L)
+            for (int N = 0; N <= L; ++N)
+                for (int t = N; t >= 0; --t)
+                    for (int u = N - t; u >= 0; --u) list[L].push_back({t, u, N - t - u});
+        // list[L] is a prefix of list[L+1]: enumeration by order N
+        dense.assign((size_t)(LM + 1) * (LM + 1) * (LM + 1), -1);
+        for (size_t i = 0; i < list[LM].size(); ++i) {
+            const auto& q = list[LM][i];
+            dense[((size_t)q[0] * (LM + 1) + q[1]) * (LM + 1) + q[2]] = (int)i;
+        }
+    }
+    inline int at(int t, int u, int v) const {
+        return dense[((size_t)t * (LM + 1) + u) * (LM + 1) + v];
+    }
+};
+const HermIndex& hidx() {
+    static HermIndex h;
+    return h;
+}
+
+// R^0_{tuv}(alpha, X, Y, Z) for t+u+v <= L, stored by position in hidx().list[L].
+// work must hold (L+1) * nherm(L) doubles.
+void compute_R(int L, double alpha, double X, double Y, double Z, double* R0, double* work) {
+    const HermIndex& H = hidx();
+    const int nh = nherm(L);
+    double F[NBOYS + 2];
+    boys().eval(L, alpha * (X * X + Y * Y + Z * Z), F);
+    // work[n * nh + idx]
+    double m2a = 1.0;
+    for (int n = 0; n <= L; ++n) {
+        work[(size_t)n * nh + 0] = m2a * F[n];
+        m2a *= -2.0 * alpha;
+    }
+    const auto& lst = H.list[L];
+    for (int i = 1; i < nh; ++i) {
+        const int t = lst[i][0], u = lst[i][1], v = lst[i][2];
+        const int N = t + u + v;
+        for (int n = 0; n <= L - N; ++n) {
+            const double* Rn1 = work + (size_t)(n + 1) * nh;
+            double val;
+            if (t > 0) {
+                val = X * Rn1[H.at(t - 1, u, v)];
+                if (t > 1) val += (t - 1) * Rn1[H.at(t - 2, u, v)];
+            } else if (u > 0) {
+                val = Y * Rn1[H.at(t, u - 1, v)];
+                if (u > 1) val += (u - 1) * Rn1[H.at(t, u - 2, v)];
+            } else {
+                val = Z * Rn1[H.at(t, u, v - 1)];
+                if (v > 1) val += (v - 1) * Rn1[H.at(t, u, v - 2)];
+            }
+            work[(size_t)n * nh + i] = val;
+        }
+    }
+    std::memcpy(R0, work, sizeof(double) * nh);
+}
+
+// ---------------------------------------------------------------- basis
+struct Shell {
+    int l;
+    double A[3];
+    std::vector<double> alpha, coef;
+    int64_t off;
+};
+
+}  // namespace
+
+struct f2ints_basis {
+    std::vector<Shell> shells;
+    int64_t nbf = 0;
+    int maxl = 0;
+};
+
+namespace {
+
+// 1-D Hermite coefficients E^{ij}_t, i <= imax, j <= jmax
+struct E1D {
+    static constexpr int D = LMAX + 3;
+    double e[D][D][2 * D];
+    void compute(int imax, int jmax, double a, double b, double Ax, double Bx) {
+        const double p = a + b, mu = a * b / p, P = (a * Ax + b * Bx) / p;
+        const double XPA = P - Ax, XPB = P - Bx, o2p = 0.5 / p;
+        for (int i = 0; i <= imax; ++i)
+            for (int j = 0; j <= jmax; ++j)
+                for (int t = 0; t <= i + j + 1; ++t) e[i][j][t] = 0.0;
+        e[0][0][0] = std::exp(-mu * (Ax - Bx) * (Ax - Bx));
+        for (int i = 0; i < imax; ++i)
+            for (int t = 0; t <= i + 1; ++t) {
+                double v = XPA * e[i][0][t];
+                if (t > 0) v += o2p * e[i][0][t - 1];
+                if (t + 1 <= i) v += (t + 1) * e[i][0][t + 1];
+                e[i + 1][0][t] = v;
+            }
+        for (int i = 0; i <= imax; ++i)
+            for (int j = 0; j < jmax; ++j)
+                for (int t = 0; t <= i + j + 1; ++t) {
+                    double v = XPB * e[i][j][t];
+                    if (t > 0) v += o2p * e[i][j][t - 1];
+                    if (t + 1 <= i + j) v += (t + 1) * e[i][j][t + 1];
+                    e[i][j + 1][t] = v;
+                }
+    }
+};
+
+// Hermite expansion of a primitive pair in the solid-harmonic product basis:
+//   E[(ma*nsb + mb) * nh + h]  (h over hidx().list[la+lb]),
+// with coefficient product and the Hermite sign (-1)^{t+u+v} when `ket`.
+struct PrimPair {
+    double p, P[3], scale;  // scale: ca*cb (exp factor is inside E)
+    std::vector<double> E;
+};
+
+struct ShellPair {
+    int a, b;
+    std::vector<PrimPair> prims;
+};
+
+void make_pair(const Shell& A, const Shell& B, bool ket, ShellPair& sp) {
+    const Harmonics& Hm = harm();
+    const HermIndex& H = hidx();
+    const int la = A.l, lb = B.l, L = la + lb, nh = nherm(L);
+    const int nca = ncart(la), ncb = ncart(lb), nsa = nsph(la), nsb = nsph(lb);
+    const double AB2 = (A.A[0] - B.A[0]) * (A.A[0] - B.A[0]) + (A.A[1] - B.A[1]) * (A.A[1] - B.A[1]) +
+                       (A.A[2] - B.A[2]) * (A.A[2] - B.A[2]);
+    E1D ex, ey, ez;
+    std::vector<double> ecart((size_t)nca * ncb * nh), etmp((size_t)nsa * ncb * nh);
+    sp.prims.clear();
+    for (size_t ia = 0; ia < A.alpha.size(); ++ia)
+        for (size_t ib = 0; ib < B.alpha.size(); ++ib) {
+            const double a = A.alpha[ia], b = B.alpha[ib], p = a + b;
+            const double cc = A.coef[ia] * B.coef[ib];
+            const double K = std::exp(-a * b / p * AB2);
+            if (std::fabs(cc) * K * std::pow(PI / p, 1.5) < kPrimScreen) continue;
+            PrimPair pp;
+            pp.p = p;
+            for (int d = 0; d < 3; ++d) pp.P[d] = (a * A.A[d] + b * B.A[d]) / p;
+            pp.scale = cc;
+            ex.compute(la, lb, a, b, A.A[0], B.A[0]);
+            ey.compute(la, lb, a, b, A.A[1], B.A[1]);
+            ez.compute(la, lb, a, b, A.A[2], B.A[2]);
+            // Cartesian products
+            for (int ca = 0; ca < nca; ++ca)
+                for (int cb = 0; cb < ncb; ++cb) {
+                    const CartComp& qa = Hm.cart[la][ca];
+                    const CartComp& qb = Hm.cart[lb][cb];
+                    double* dst = &ecart[((size_t)ca * ncb + cb) * nh];
+                    for (int h = 0; h < nh; ++h) {
+                        const auto& tuv = H.list[L][h];
+                        if (tuv[0] > qa.x + qb.x || tuv[1] > qa.y + qb.y || tuv[2] > qa.z + qb.z) {
+                            dst[h] = 0.0;
+                            continue;
+                        }
+                        double v = ex.e[qa.x][qb.x][tuv[0]] * ey.e[qa.y][qb.y][tuv[1]] *
+                                   ez.e[qa.z][qb.z][tuv[2]];
+                        if (ket && ((tuv[0] + tuv[1] + tuv[2]) & 1)) v = -v;
+                        dst[h] = v;
+                    }
+                }
+            // transform a, then b
+            std::fill(etmp.begin(), etmp.end(), 0.0);
+            for (int sa = 0; sa < nsa; ++sa)
+                for (const SphTerm& ta : Hm.sparse[la][sa])
+                    for (int cb = 0; cb < ncb; ++cb) {
+                        const double* src = &ecart[((size_t)ta.cart * ncb + cb) * nh];
+                        double* dst = &etmp[((size_t)sa * ncb + cb) * nh];
+                        for (int h = 0; h < nh; ++h) dst[h] += ta.c * src[h];
+                    }
+            pp.E.assign((size_t)nsa * nsb * nh, 0.0);
+            for (int sa = 0; sa < nsa; ++sa)
+                for (int sb = 0; sb < nsb; ++sb)
+                    for (const SphTerm& tb : Hm.sparse[lb][sb]) {
+                        const double* src = &etmp[((size_t)sa * ncb + tb.cart) * nh];
+                        double* dst = &pp.E[((size_t)sa * nsb + sb) * nh];
+                        for (int h = 0; h < nh; ++h) dst[h] += tb.c * src[h];
+                    }
+            sp.prims.push_back(std::move(pp));
+        }
+}
+
+int nthreads() { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
+
+// ---------------------------------------------------------------- one-electron
+enum OneE { kOverlap, kKinetic };
+
+int one_electron(const f2ints_basis* bs, OneE kind, double* out) {
+    const Harmonics& Hm = harm();
+    const int ns = (int)bs->shells.size();
+    const int64_t nbf = bs->nbf;
+    std::vector<std::pair<int, int>> pairs;
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) pairs.push_back({a, b});
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
+    for (size_t q = 0; q < pairs.size(); ++q) {
+        const Shell& A = bs->shells[pairs[q].first];
+        const Shell& B = bs->shells[pairs[q].second];
+        const int la = A.l, lb = B.l, nca = ncart(la), ncb = ncart(lb);
+        std::vector<double> cartv((size_t)nca * ncb, 0.0);
+        E1D ex, ey, ez;
+        for (size_t ia = 0; ia < A.alpha.size(); ++ia)
+            for (size_t ib = 0; ib < B.alpha.size(); ++ib) {
+                const double a = A.alpha[ia], b = B.alpha[ib], p = a + b;
+                const double cc = A.coef[ia] * B.coef[ib] * std::pow(PI / p, 1.5);
+                const int jx = (kind == kKinetic) ? lb + 2 : lb;
+                ex.compute(la, jx, a, b, A.A[0], B.A[0]);
+                ey.compute(la, jx, a, b, A.A[1], B.A[1]);
+                ez.compute(la, jx, a, b, A.A[2], B.A[2]);
+                for (int ca = 0; ca < nca; ++ca)
+                    for (int cb = 0; cb < ncb; ++cb) {
+                        const CartComp& qa = Hm.cart[la][ca];
+                        const CartComp& qb = Hm.cart[lb][cb];
+                        const double sx = ex.e[qa.x][qb.x][0], sy = ey.e[qa.y][qb.y][0],
+                                     sz = ez.e[qa.z][qb.z][0];
+                        double v;
+                        if (kind == kOverlap) {
+                            v = sx * sy * sz;
+                        } else {
+                            // -1/2 d^2/dx^2 acting on x^j e^{-b x^2}
+                            auto t1 = [&](const E1D& e, int i, int j) {
+                                double r = -2.0 * b * b * e.e[i][j + 2][0] + b * (2 * j + 1) * e.e[i][j][0];
+                                if (j >= 2) r -= 0.5 * j * (j - 1) * e.e[i][j - 2][0];
+                                return r;
+                            };
+                            v = t1(ex, qa.x, qb.x) * sy * sz + sx * t1(ey, qa.y, qb.y) * sz +
+                                sx * sy * t1(ez, qa.z, qb.z);
+                        }
+                        cartv[(size_t)ca * ncb + cb] += cc * v;
+                    }
+            }
+        const int nsa = nsph(la), nsb = nsph(lb);
+        for (int sa = 0; sa < nsa; ++sa)
+            for (int sb = 0; sb < nsb; ++sb) {
+                double v = 0.0;
+                for (const SphTerm& ta : Hm.sparse[la][sa])
+                    for (const SphTerm& tb : Hm.sparse[lb][sb]) v += ta.c * tb.c * cartv[(size_t)ta.cart * ncb + tb.cart];
+                const int64_t m = A.off + sa, n = B.off + sb;
+                out[m * nbf + n] = v;
+                out[n * nbf + m] = v;
+            }
+    }
+    return 0;
+}
+
+int nuclear(const f2ints_basis* bs, int nc, const double* Z, const double* xyz, double* out) {
+    const HermIndex& H = hidx();
+    const int ns = (int)bs->shells.size();
+    const int64_t nbf = bs->nbf;
+    std::vector<std::pair<int, int>> pairs;
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b) pairs.push_back({a, b});
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
+    for (size_t q = 0; q < pairs.size(); ++q) {
+        const Shell& A = bs->shells[pairs[q].first];
+        const Shell& B = bs->shells[pairs[q].second];
+        ShellPair sp;
+        make_pair(A, B, false, sp);
+        const int L = A.l + B.l, nh = nherm(L), nsa = nsph(A.l), nsb = nsph(B.l);
+        std::vector<double> R(nh), work((size_t)(L + 1) * nh), acc((size_t)nsa * nsb, 0.0);
+        for (const PrimPair& pp : sp.prims) {
+            for (int c = 0; c < nc; ++c) {
+                const double X = pp.P[0] - xyz[3 * c], Y = pp.P[1] - xyz[3 * c + 1],
+                             Zc = pp.P[2] - xyz[3 * c + 2];
+                compute_R(L, pp.p, X, Y, Zc, R.data(), work.data());
+                const double pref = -Z[c] * 2.0 * PI / pp.p * pp.scale;
+                for (int s = 0; s < nsa * nsb; ++s) {
+                    const double* e = &pp.E[(size_t)s * nh];
+                    double v = 0.0;
+                    for (int h = 0; h < nh; ++h) v += e[h] * R[h];
+                    acc[s] += pref * v;
+                }
+            }
+        }
+        (void)H;
+        for (int sa = 0; sa < nsa; ++sa)
+            for (int sb = 0; sb < nsb; ++sb) {
+                const int64_t m = A.off + sa, n = B.off + sb;
+                out[m * nbf + n] = acc[(size_t)sa * nsb + sb];
+                out[n * nbf + m] = acc[(size_t)sa * nsb + sb];
+            }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- 2-centre Coulomb
+int coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* out) {
+    const Harmonics& Hm = harm();
+    const HermIndex& H = hidx();
+    const bool same = (a == b);
+    const int na = (int)a->shells.size(), nb = (int)b->shells.size();
+    const int64_t nbfb = b->nbf;
+    std::vector<std::pair<int, int>> pairs;
+    for (int i = 0; i < na; ++i)
+        for (int j = 0; j < (same ? i + 1 : nb); ++j) pairs.push_back({i, j});
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
+    for (size_t q = 0; q < pairs.size(); ++q) {
+        const Shell& P = a->shells[pairs[q].first];
+        const Shell& Q = b->shells[pairs[q].second];
+        const int L = P.l + Q.l, nh = nherm(L), nsP = nsph(P.l), nsQ = nsph(Q.l);
+        std::vector<double> R(nh), work((size_t)(L + 1) * nh), acc((size_t)nsP * nsQ, 0.0);
+        const double X = P.A[0] - Q.A[0], Y = P.A[1] - Q.A[1], Z = P.A[2] - Q.A[2];
+        const double sgnQ = (Q.l & 1) ? -1.0 : 1.0;
+        for (size_t ip = 0; ip < P.alpha.size(); ++ip)
+            for (size_t iq = 0; iq < Q.alpha.size(); ++iq) {
+                const double g = P.alpha[ip], d = Q.alpha[iq];
+                const double al = g * d / (g + d);
+                compute_R(L, al, X, Y, Z, R.data(), work.data());
+                const double pref = 2.0 * std::pow(PI, 2.5) / (g * d * std::sqrt(g + d)) * P.coef[ip] *
+                                    Q.coef[iq] * std::pow(2.0 * g, -P.l) * std::pow(2.0 * d, -Q.l) * sgnQ;
+                for (int sP = 0; sP < nsP; ++sP)
+                    for (int sQ = 0; sQ < nsQ; ++sQ) {
+                        double v = 0.0;
+                        for (const SphTerm& tp : Hm.sparse[P.l][sP]) {
+                            const CartComp& cp = Hm.cart[P.l][tp.cart];
+                            for (const SphTerm& tq : Hm.sparse[Q.l][sQ]) {
+                                const CartComp& cq = Hm.cart[Q.l][tq.cart];
+                                v += tp.c * tq.c * R[H.at(cp.x + cq.x, cp.y + cq.y, cp.z + cq.z)];
+                            }
+                        }
+                        acc[(size_t)sP * nsQ + sQ] += pref * v;
+                    }
+            }
+        for (int sP = 0; sP < nsP; ++sP)
+            for (int sQ = 0; sQ < nsQ; ++sQ) {
+                const int64_t m = P.off + sP, n = Q.off + sQ;
+                out[m * nbfb + n] = acc[(size_t)sP * nsQ + sQ];
+                if (same) out[n * nbfb + m] = acc[(size_t)sP * nsQ + sQ];
+            }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- 3-centre Coulomb
+int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1, double* out) {
+    const Harmonics& Hm = harm();
+    const HermIndex& H = hidx();
+    const int nsb = (int)bs->shells.size();
+    const int64_t nbf = bs->nbf;
+    const int64_t first = aux->shells[sh0].off;
+    std::vector<std::pair<int, int>> pairs;
+    for (int a = 0; a < nsb; ++a)
+        for (int b = 0; b <= a; ++b) pairs.push_back({a, b});
+    const int64_t nP = (sh1 < (int)aux->shells.size() ? aux->shells[sh1].off : aux->nbf) - first;
+    std::fill(out, out + (size_t)nP * nbf * nbf, 0.0);
+    int maxlaux = 0;
+    for (int s = sh0; s < sh1; ++s) maxlaux = std::max(maxlaux, aux->shells[s].l);
+    const int Lmax = maxlaux + 2 * bs->maxl;
+#pragma omp parallel num_threads(nthreads())
+    {
+        std::vector<double> R(nherm(Lmax)), work((size_t)(Lmax + 1) * nherm(Lmax));
+        std::vector<double> Y((size_t)nsph(maxlaux) * nherm(2 * bs->maxl));
+        std::vector<double> acc;
+        ShellPair sp;
+#pragma omp for schedule(dynamic)
+        for (size_t q = 0; q < pairs.size(); ++q) {
+            const Shell& A = bs->shells[pairs[q].first];
+            const Shell& B = bs->shells[pairs[q].second];
+            make_pair(A, B, true, sp);
+            if (sp.prims.empty()) continue;
+            const int lab = A.l + B.l, nhab = nherm(lab), nsa = nsph(A.l), nsbb = nsph(B.l);
+            const int nab = nsa * nsbb;
+            for (int s = sh0; s < sh1; ++s) {
+                const Shell& P = aux->shells[s];
+                const int lP = P.l, nsP = nsph(lP), L = lP + lab, nhL = nherm(L);
+                acc.assign((size_t)nsP * nab, 0.0);
+                for (size_t k = 0; k < P.alpha.size(); ++k) {
+                    const double g = P.alpha[k];
+                    for (const PrimPair& pp : sp.prims) {
+                        const double p = pp.p, al = g * p / (g + p);
+                        compute_R(L, al, P.A[0] - pp.P[0], P.A[1] - pp.P[1], P.A[2] - pp.P[2], R.data(),
+                                  work.data());
+                        const double pref = 2.0 * std::pow(PI, 2.5) / (g * p * std::sqrt(g + p)) *
+                                            P.coef[k] * std::pow(2.0 * g, -lP) * pp.scale;
+                        // Y[sP][h] = sum_c T_{lP}(sP, c) R[c + h]
+                        for (int sP = 0; sP < nsP; ++sP) {
+                            double* y = &Y[(size_t)sP * nhab];
+                            std::fill(y, y + nhab, 0.0);
+                            for (const SphTerm& tp : Hm.sparse[lP][sP]) {
+                                const CartComp& cp = Hm.cart[lP][tp.cart];
+                                for (int h = 0; h < nhab; ++h) {
+                                    const auto& tuv = H.list[lab][h];
+                                    y[h] += tp.c * R[H.at(cp.x + tuv[0], cp.y + tuv[1], cp.z + tuv[2])];
+                                }
+                            }
+                        }
+                        (void)nhL;
+                        for (int sP = 0; sP < nsP; ++sP) {
+                            const double* y = &Y[(size_t)sP * nhab];
+                            double* a = &acc[(size_t)sP * nab];
+                            for (int s2 = 0; s2 < nab; ++s2) {
+                                const double* e = &pp.E[(size_t)s2 * nhab];
+                                double v = 0.0;
+                                for (int h = 0; h < nhab; ++h) v += e[h] * y[h];
+                                a[s2] += pref * v;
+                            }
+                        }
+                    }
+                }
+                for (int sP = 0; sP < nsP; ++sP) {
+                    double* o = out + (size_t)(P.off + sP - first) * nbf * nbf;
+                    for (int sa = 0; sa < nsa; ++sa)
+                        for (int sb = 0; sb < nsbb; ++sb) {
+                            const double v = acc[(size_t)sP * nab + sa * nsbb + sb];
+                            const int64_t m = A.off + sa, n = B.off + sb;
+                            o[m * nbf + n] = v;
+                            o[n * nbf + m] = v;
+                        }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- 4-centre (tests)
+int coulomb_4c(const f2ints_basis* bs, double* out) {
+    const HermIndex& H = hidx();
+    const int ns = (int)bs->shells.size();
+    const int64_t n = bs->nbf;
+    std::vector<ShellPair> bra(ns * ns), ket(ns * ns);
+    for (int a = 0; a < ns; ++a)
+        for (int b = 0; b < ns; ++b) {
+            make_pair(bs->shells[a], bs->shells[b], false, bra[a * ns + b]);
+            make_pair(bs->shells[a], bs->shells[b], true, ket[a * ns + b]);
+        }
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
+    for (int ab = 0; ab < ns * ns; ++ab) {
+        const Shell& A = bs->shells[ab / ns];
+        const Shell& B = bs->shells[ab % ns];
+        const int lab = A.l + B.l, nhab = nherm(lab), nsa = nsph(A.l), nsbb = nsph(B.l);
+        for (int cd = 0; cd < ns * ns; ++cd) {
+            const Shell& C = bs->shells[cd / ns];
+            const Shell& D = bs->shells[cd % ns];
+            const int lcd = C.l + D.l, nhcd = nherm(lcd), nsc = nsph(C.l), nsd = nsph(D.l);
+            const int L = lab + lcd;
+            std::vector<double> R(nherm(L)), work((size_t)(L + 1) * nherm(L));
+            std::vector<double> acc((size_t)nsa * nsbb * nsc * nsd, 0.0);
+            for (const PrimPair& p1 : bra[ab].prims)
+                for (const PrimPair& p2 : ket[cd].prims) {
+                    const double p = p1.p, qq = p2.p, al = p * qq / (p + qq);
+                    compute_R(L, al, p1.P[0] - p2.P[0], p1.P[1] - p2.P[1], p1.P[2] - p2.P[2], R.data(),
+                              work.data());
+                    const double pref = 2.0 * std::pow(PI, 2.5) / (p * qq * std::sqrt(p + qq)) * p1.scale * p2.scale;
+                    for (int s1 = 0; s1 < nsa * nsbb; ++s1)
+                        for (int s2 = 0; s2 < nsc * nsd; ++s2) {
+                            double v = 0.0;
+                            for (int h1 = 0; h1 < nhab; ++h1) {
+                                const double e1 = p1.E[(size_t)s1 * nhab + h1];
+                                if (e1 == 0.0) continue;
+                                const auto& t1 = H.list[lab][h1];
+                                double w = 0.0;
+                                for (int h2 = 0; h2 < nhcd; ++h2) {
+                                    const auto& t2 = H.list[lcd][h2];
+                                    w += p2.E[(size_t)s2 * nhcd + h2] * R[H.at(t1[0] + t2[0], t1[1] + t2[1], t1[2] + t2[2])];
+                                }
+                                v += e1 * w;
+                            }
+                            acc[(size_t)s1 * nsc * nsd + s2] += pref * v;
+                        }
+                }
+            for (int sa = 0; sa < nsa; ++sa)
+                for (int sb = 0; sb < nsbb; ++sb)
+                    for (int sc = 0; sc < nsc; ++sc)
+                        for (int sd = 0; sd < nsd; ++sd) {
+                            const int64_t i = A.off + sa, j = B.off + sb, k = C.off + sc, l = D.off + sd;
+                            out[((i * n + j) * n + k) * n + l] =
+                                acc[((size_t)(sa * nsbb + sb)) * nsc * nsd + sc * nsd + sd];
+                        }
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+int f2ints_basis_create(int nshell, const int* l, const int* nprim, const double* xyz, const double* exps,
+                        const double* coefs, f2ints_basis** out) {
+    if (!out || nshell < 0 || (nshell > 0 && (!l || !nprim || !xyz || !exps || !coefs)))
+        return fail("f2ints_basis_create: invalid arguments");
+    f2ints_basis* b = new f2ints_basis();
+    int64_t off = 0, k = 0;
+    for (int s = 0; s < nshell; ++s) {
+        if (l[s] < 0 || l[s] > LMAX || nprim[s] < 1) {
+            delete b;
+            return fail("f2ints_basis_create: shell " + std::to_string(s) + " has l or nprim out of range");
+        }
+        Shell sh;
+        sh.l = l[s];
+        for (int d = 0; d < 3; ++d) sh.A[d] = xyz[3 * s + d];
+        for (int p = 0; p < nprim[s]; ++p, ++k) {
+            if (coefs[k] == 0.0) continue;  // zero columns of general contractions
+            sh.alpha.push_back(exps[k]);
+            sh.coef.push_back(coefs[k]);
+        }
+        if (sh.alpha.empty()) {  // keep one zero primitive so the shell exists
+            sh.alpha.push_back(exps[k - 1]);
+            sh.coef.push_back(0.0);
+        }
+        sh.off = off;
+        off += nsph(sh.l);
+        b->maxl = std::max(b->maxl, sh.l);
+        b->shells.push_back(std::move(sh));
+    }
+    b->nbf = off;
+    *out = b;
+    return 0;
+}
+
+int f2ints_basis_destroy(f2ints_basis* b) {
+    delete b;
+    return 0;
+}
+
+int64_t f2ints_basis_nbf(const f2ints_basis* b) { return b ? b->nbf : -1; }
+
+int f2ints_max_l(void) { return LMAX; }
+
+int f2ints_overlap(const f2ints_basis* b, double* S) {
+    if (!b || !S) return fail("f2ints_overlap: null argument");
+    return one_electron(b, kOverlap, S);
+}
+
+int f2ints_kinetic(const f2ints_basis* b, double* T) {
+    if (!b || !T) return fail("f2ints_kinetic: null argument");
+    return one_electron(b, kKinetic, T);
+}
+
+int f2ints_nuclear(const f2ints_basis* b, int nc, const double* Z, const double* xyz, double* V) {
+    if (!b || !V || nc < 0 || (nc > 0 && (!Z || !xyz))) return fail("f2ints_nuclear: invalid arguments");
+    return nuclear(b, nc, Z, xyz, V);
+}
+
+double f2ints_nuclear_repulsion(int natom, const double* Z, const double* xyz) {
+    double e = 0.0;
+    for (int i = 0; i < natom; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                         dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            e += Z[i] * Z[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    return e;
+}
+
+int f2ints_coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* M) {
+    if (!a || !b || !M) return fail("f2ints_coulomb_2c: null argument");
+    return coulomb_2c(a, b, M);
+}
+
+int f2ints_coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sb, int se, double* out) {
+    if (!aux || !bs || !out) return fail("f2ints_coulomb_3c: null argument");
+    const int ns = (int)aux->shells.size();
+    if (se < 0 || se > ns) se = ns;
+    if (sb < 0 || sb > se) return fail("f2ints_coulomb_3c: bad shell range");
+    if (sb == se) return 0;
+    return coulomb_3c(aux, bs, sb, se, out);
+}
+
+int f2ints_coulomb_4c(const f2ints_basis* b, double* out) {
+    if (!b || !out) return fail("f2ints_coulomb_4c: null argument");
+    if (b->nbf > 200) return fail("f2ints_coulomb_4c: basis too large for the test-only 4-centre routine");
+    return coulomb_4c(b, out);
+}
+
+int f2ints_boys(int nmax, double T, double* out) {
+    if (nmax < 0 || nmax > NBOYS || !out || T < 0) return fail("f2ints_boys: invalid arguments");
+    boys().eval(nmax, T, out);
+    return 0;
+}
+
+int f2ints_cart2sph(int l, double* out) {
+    if (l < 0 || l > LMAX || !out) return fail("f2ints_cart2sph: invalid arguments");
+    const auto& T = harm().T[l];
+    std::copy(T.begin(), T.end(), out);
+    return 0;
+}
+
+int f2ints_set_threads(int n) {
+    g_threads = n;
+    return 0;
+}
+
+const char* f2ints_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+"""Host integrals through libf2ints.so (numpy in, numpy out).
+
+Mirrors forte2/integrals/integrals.py: overlap (:198), kinetic (:224),
+nuclear (:250), nuclear_repulsion (:156), coulomb_2c (:903-934),
+coulomb_3c (:811-862) and the test-only coulomb_4c.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _lib():
+    return _native.ints_lib()
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(_lib().f2ints_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def overlap(basis):
+    n = basis.size
+    S = np.empty((n, n))
+    _check(_lib().f2ints_overlap(basis.handle, _ptr(S)))
+    return S
+
+
+def kinetic(basis):
+    n = basis.size
+    T = np.empty((n, n))
+    _check(_lib().f2ints_kinetic(basis.handle, _ptr(T)))
+    return T
+
+
+def nuclear(basis, atoms):
+    n = basis.size
+    V = np.empty((n, n))
+    Z = np.ascontiguousarray([float(z) for z, _ in atoms])
+    xyz = np.ascontiguousarray(np.array([p for _, p in atoms], dtype=float).reshape(-1))
+    _check(_lib().f2ints_nuclear(basis.handle, len(atoms), _ptr(Z), _ptr(xyz), _ptr(V)))
+    return V
+
+
+def nuclear_repulsion(atoms):
+    Z = np.ascontiguousarray([float(z) for z, _ in atoms])
+    xyz = np.ascontiguousarray(np.array([p for _, p in atoms], dtype=float).reshape(-1))
+    return float(_lib().f2ints_nuclear_repulsion(len(atoms), _ptr(Z), _ptr(xyz)))
+
+
+def coulomb_2c(basis1, basis2=None):
+    b2 = basis1 if basis2 is None else basis2
+    M = np.empty((basis1.size, b2.size))
+    _check(_lib().f2ints_coulomb_2c(basis1.handle, b2.handle, _ptr(M)))
+    return M
+
+
+def coulomb_3c(aux, basis, shell_begin=0, shell_end=None, out=None):
+    """(P|mn) for aux shells [shell_begin, shell_end): array (nP, nbf, nbf)."""
+    offs = aux.shell_offsets()
+    se = aux.nshells if shell_end is None else shell_end
+    nP = offs[se] - offs[shell_begin]
+    n = basis.size
+    if out is None:
+        out = np.empty((nP, n, n))
+    elif out.shape != (nP, n, n) or not out.flags.c_contiguous or out.dtype != np.float64:
+        raise ValueError("out has the wrong shape/layout")
+    _check(_lib().f2ints_coulomb_3c(aux.handle, basis.handle, shell_begin, se, _ptr(out)))
+    return out
+
+
+def coulomb_4c(basis):
+    n = basis.size
+    out = np.empty((n, n, n, n))
+    _check(_lib().f2ints_coulomb_4c(basis.handle, _ptr(out)))
+    return out
+
+
+def boys(nmax, T):
+    out = np.empty(nmax + 1)
+    _check(_lib().f2ints_boys(nmax, float(T), _ptr(out)))
+    return out
+
+
+def cart2sph(l):
+    out = np.empty((2 * l + 1, (l + 1) * (l + 2) // 2))
+    _check(_lib().f2ints_cart2sph(l, _ptr(out)))
+    return out
+
+
+def set_threads(n):
+    _lib().f2ints_set_threads(int(n))

@@ -1,0 +1,99 @@
+"""Molecular system: geometry, basis sets, one-time host integrals.
+
+Mirrors the parts of forte2's ``System`` (forte2/system/system.py:30-200,
+405-484) that the SCF loops use: ``atoms``, ``Zsum``, ``nbf``, ``naux``,
+``nmo``, ``nuclear_repulsion``, ``ints_overlap``, ``ints_hcore``,
+``get_Xorth`` and the ``fock_builder`` slot (a device FockBuilder, created on
+first use).  Integrals come from the host engine in libf2ints.so.
+"""
+
+import numpy as np
+
+from . import basis as _basis
+from . import geometry as _geom
+from . import integrals as _ints
+
+
+def canonical_orth(S, rtol=1e-8):
+    """Canonical orthogonalisation X = U s^-1/2 over eigenvalues >= rtol*max
+    (forte2/helpers/matrix_functions.py:95-152).  Returns (X, n_kept)."""
+    w, U = np.linalg.eigh(S)
+    if np.any(w < -1e-14):
+        raise ValueError("The metric matrix must be positive semi-definite.")
+    w[w < 0] = 0.0
+    nd = int(np.searchsorted(w, rtol * w[-1]))
+    return U[:, nd:] / np.sqrt(w[nd:]), len(w) - nd
+
+
+class System:
+    def __init__(self, xyz, basis_set, auxiliary_basis_set=None, unit="angstrom",
+                 overlap_ortho_rtol=1e-8, df_ortho_rtol=None, device=None):
+        self.xyz = xyz
+        self.unit = unit
+        self.atoms = _geom.parse_geometry(xyz, unit) if isinstance(xyz, str) else list(xyz)
+        self.Zsum = sum(z for z, _ in self.atoms)
+        self.basis_set = basis_set
+        self.auxiliary_basis_set = auxiliary_basis_set
+        self.basis = _basis.build_basis(basis_set, self.atoms)
+        self.auxiliary_basis = (_basis.build_basis(auxiliary_basis_set, self.atoms)
+                                if auxiliary_basis_set is not None else None)
+        self.nbf = self.basis.size
+        self.naux = self.auxiliary_basis.size if self.auxiliary_basis is not None else 0
+        self.df_ortho_rtol = df_ortho_rtol
+        self.overlap_ortho_rtol = overlap_ortho_rtol
+        self.nuclear_repulsion = _ints.nuclear_repulsion(self.atoms)
+        self._S = None
+        self.Xorth, self.nmo = canonical_orth(self.ints_overlap(), overlap_ortho_rtol)
+        self.two_component = False
+        self.device = device
+        self._fock_builder = None
+
+    # ------------------------------------------------------------------ integrals
+    def ints_overlap(self):
+        if self._S is None:
+            self._S = _ints.overlap(self.basis)
+        return self._S.copy()
+
+    def ints_kinetic(self):
+        return _ints.kinetic(self.basis)
+
+    def ints_nuclear(self):
+        return _ints.nuclear(self.basis, self.atoms)
+
+    def ints_hcore(self):
+        return self.ints_kinetic() + self.ints_nuclear()
+
+    def get_Xorth(self):
+        return self.Xorth
+
+    def coulomb_2c(self):
+        if self.auxiliary_basis is None:
+            raise ValueError("Auxiliary basis is not defined.")
+        return _ints.coulomb_2c(self.auxiliary_basis)
+
+    def coulomb_3c(self, shell_begin=0, shell_end=None, out=None):
+        if self.auxiliary_basis is None:
+            raise ValueError("Auxiliary basis is not defined.")
+        return _ints.coulomb_3c(self.auxiliary_basis, self.basis, shell_begin, shell_end, out)
+
+    def sap_potential(self):
+        """V_SAP = sum_P (P|mn) over the SAP charge-density basis (scf_utils.py:10-51)."""
+        dens = _basis.build_sap_density_basis(self.atoms)
+        V = np.zeros((self.nbf, self.nbf))
+        step = 64
+        for s0 in range(0, dens.nshells, step):
+            V += _ints.coulomb_3c(dens, self.basis, s0, min(dens.nshells, s0 + step)).sum(axis=0)
+        return V
+
+    # ------------------------------------------------------------------ Fock builder slot
+    @property
+    def fock_builder(self):
+        if self._fock_builder is None:
+            from .jkbuilder import FockBuilder
+
+            self._fock_builder = FockBuilder(self, device=self.device)
+        return self._fock_builder
+
+    @fock_builder.setter
+    def fock_builder(self, fb):
+        self._fock_builder = fb

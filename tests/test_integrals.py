@@ -1,0 +1,148 @@
+"""Host integral engine (libf2ints.so) against the reference's known answers,
+closed forms and the AO convention of forte2/integrals/real_spherical_harmonics.hpp.
+
+Golden values: tests/integrals/test_two_electron.py:10-46 (25 STO-3G water
+(mn|rs)), tests/integrals/test_libcint.py:219-241,276-287 (Frobenius norms of
+(P|Q), (m|P), (P|mn)), all at the reference's tolerances (abs 5e-8, rel 1e-6).
+"""
+
+import math
+
+import mpmath
+import numpy as np
+import pytest
+
+from forte2_amd import basis as bs
+from forte2_amd import geometry as geo
+from forte2_amd import integrals as ints
+
+WATER_STO3G = """O 0.000000000000     0.000000000000    -0.061664597388
+H 0.000000000000    -0.711620616370     0.489330954643
+H 0.000000000000     0.711620616370     0.489330954643"""
+
+STO3G_ERI = {
+    (4, 5, 0, 4): 0.011203183573992602, (6, 3, 0, 1): 0.043937066018280665,
+    (6, 1, 0, 1): 0.10316476705668452, (1, 3, 1, 5): 0.0347130867494285,
+    (3, 2, 2, 5): 0.012205381062317456, (0, 4, 4, 5): 0.011203183573992604,
+    (0, 4, 5, 4): 0.011203183573992604, (2, 5, 2, 1): 0.088959722367807,
+    (1, 1, 6, 0): 0.0669877231256632, (2, 3, 3, 5): -0.013249429680865767,
+    (2, 3, 6, 3): 0.013249429680865767, (3, 5, 1, 6): 0.08488308941086266,
+    (5, 0, 0, 5): 0.009602115579028458, (1, 2, 3, 5): -0.01955131963900513,
+    (0, 2, 5, 2): 0.014154072956092055, (5, 5, 5, 1): 0.33820230329917145,
+    (1, 3, 0, 5): 0.0018694062463856086, (0, 0, 0, 0): 4.785065751815717,
+    (0, 5, 3, 0): 0.0010038501294464502, (3, 6, 1, 1): 0.163693540728617,
+    (3, 3, 2, 2): 0.785270203138278, (3, 2, 3, 6): 0.013249429680865767,
+    (0, 3, 1, 5): 0.004687753192919011, (1, 0, 6, 2): 0.056745507084677765,
+    (0, 5, 5, 3): 0.012314281459575225,
+}
+
+
+def test_sto3g_water_eri_golden():
+    b = bs.build_basis("sto-3g", geo.parse_geometry(WATER_STO3G))
+    V = ints.coulomb_4c(b)
+    for k, v in STO3G_ERI.items():
+        assert V[k] == pytest.approx(v, abs=5e-8), k
+    # 8-fold permutational symmetry
+    assert np.abs(V - V.transpose(1, 0, 2, 3)).max() < 1e-14
+    assert np.abs(V - V.transpose(2, 3, 0, 1)).max() < 1e-14
+
+
+def test_three_center_norm_golden():
+    atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5")
+    P = ints.coulomb_3c(bs.build_basis("cc-pvtz-jkfit", atoms), bs.build_basis("cc-pvdz", atoms))
+    assert np.linalg.norm(P) == pytest.approx(60.4085268377979, rel=1e-6)
+    assert np.abs(P - P.transpose(0, 2, 1)).max() == 0.0
+
+
+def test_two_center_norms_golden():
+    atoms = geo.parse_geometry("N 0 0 0\nN 0 0 1.1")
+    b, a = bs.build_basis("cc-pvdz", atoms), bs.build_basis("cc-pvtz-jkfit", atoms)
+    M = ints.coulomb_2c(a)
+    assert np.linalg.norm(M) == pytest.approx(159.31789654133004, rel=1e-6)
+    assert np.linalg.norm(ints.coulomb_2c(b, a)) == pytest.approx(149.6527772441834, rel=1e-6)
+    assert np.linalg.eigvalsh(M).min() > 0  # Coulomb metric is positive definite
+
+
+@pytest.mark.parametrize("T", [0.0, 1e-9, 0.013, 0.7, 4.4, 17.3, 29.99, 35.99, 36.01, 80.0, 500.0])
+def test_boys_function_against_mpmath(T):
+    mpmath.mp.dps = 40
+    F = ints.boys(24, T)
+    for n in range(25):
+        if T == 0.0:
+            ref = 1.0 / (2 * n + 1)
+        else:
+            a = mpmath.mpf(n) + mpmath.mpf(1) / 2
+            ref = float(mpmath.gammainc(a, 0, T) / (2 * mpmath.mpf(T) ** a))
+        assert abs(F[n] - ref) <= 2e-15 * ref, (n, T)
+
+
+def test_s_type_closed_forms():
+    # two normalised s primitives: S = (4ab/(a+b)^2)^{3/4} e^{-ab/(a+b) R^2};
+    # (a|b) Coulomb of unit-normalised s Gaussians via F_0
+    a_, b_, R = 0.8, 1.7, 1.3
+    basis = bs.BasisSet()
+    basis.add(0, [a_], [1.0], [0, 0, 0])
+    basis.add(0, [b_], [1.0], [0, 0, R])
+    S = ints.overlap(basis)
+    assert S[0, 1] == pytest.approx((4 * a_ * b_ / (a_ + b_) ** 2) ** 0.75 * math.exp(-a_ * b_ / (a_ + b_) * R * R), rel=1e-14)
+    M = ints.coulomb_2c(basis)
+    Na, Nb = (2 * a_ / math.pi) ** 0.75, (2 * b_ / math.pi) ** 0.75
+    al = a_ * b_ / (a_ + b_)
+    F0 = 0.5 * math.sqrt(math.pi / (al * R * R)) * math.erf(math.sqrt(al * R * R))
+    ref = Na * Nb * 2 * math.pi ** 2.5 / (a_ * b_ * math.sqrt(a_ + b_)) * F0
+    assert M[0, 1] == pytest.approx(ref, rel=1e-14)
+    # kinetic energy of a normalised s Gaussian: 3a/2
+    assert ints.kinetic(basis)[0, 0] == pytest.approx(1.5 * a_, rel=1e-14)
+    # nuclear attraction of a unit charge at the centre: -2 sqrt(2a/pi)
+    V = ints.nuclear(basis, [(1, np.zeros(3))])
+    assert V[0, 0] == pytest.approx(-2.0 * math.sqrt(2 * a_ / math.pi), rel=1e-14)
+
+
+def _poly(T, l, x, y, z):
+    cart = [(a, b, l - a - b) for a in range(l, -1, -1) for b in range(l - a, -1, -1)]
+    mono = np.array([x ** a * y ** b * z ** c for a, b, c in cart])
+    return T @ mono
+
+
+def test_solid_harmonic_convention():
+    # AO order and phases of forte2/integrals/real_spherical_harmonics.hpp:33-60
+    rng = np.random.default_rng(0)
+    s3 = math.sqrt(3.0)
+    for x, y, z in rng.standard_normal((5, 3)):
+        assert np.allclose(_poly(ints.cart2sph(1), 1, x, y, z), [y, z, x], atol=1e-14)
+        assert np.allclose(_poly(ints.cart2sph(2), 2, x, y, z),
+                           [s3 * x * y, s3 * y * z, z * z - 0.5 * (x * x + y * y), s3 * x * z,
+                            0.5 * s3 * (x * x - y * y)], atol=1e-13)
+    # all l: harmonic (Laplacian zero), mutually orthogonal on the unit sphere
+    # with the norm of z^l, and the m = 0 component leads with z^l
+    def sphere(a, b, c):  # integral of x^a y^b z^c over the unit sphere
+        if a % 2 or b % 2 or c % 2:
+            return 0.0
+        return 2 * math.gamma((a + 1) / 2) * math.gamma((b + 1) / 2) * math.gamma((c + 1) / 2) / \
+            math.gamma((a + b + c + 3) / 2)
+
+    for l in range(1, 7):
+        T = ints.cart2sph(l)
+        cart = [(a, b, l - a - b) for a in range(l, -1, -1) for b in range(l - a, -1, -1)]
+        Mc = np.array([[sphere(p[0] + q[0], p[1] + q[1], p[2] + q[2]) for q in cart] for p in cart])
+        G = T @ Mc @ T.T
+        zl = sphere(0, 0, 2 * l)
+        assert np.abs(G - zl * np.eye(2 * l + 1)).max() < 1e-12 * zl
+        assert T[l, cart.index((0, 0, l))] == pytest.approx(1.0)
+        # exact Laplacian in the monomial basis of degree l-2
+        if l >= 2:
+            low = [(a, b, l - 2 - a - b) for a in range(l - 2, -1, -1) for b in range(l - 2 - a, -1, -1)]
+            Lap = np.zeros((len(low), len(cart)))
+            for j, (a, b, c) in enumerate(cart):
+                for k, (da, db, dc) in enumerate(((2, 0, 0), (0, 2, 0), (0, 0, 2))):
+                    e = (a, b, c)[k]
+                    if e >= 2:
+                        Lap[low.index((a - da, b - db, c - dc)), j] += e * (e - 1)
+            assert np.abs(T @ Lap.T).max() < 1e-12
+
+
+def test_contraction_normalisation():
+    atoms = geo.parse_geometry(WATER_STO3G)
+    for name in ("sto-3g", "cc-pvdz", "cc-pvqz"):
+        S = ints.overlap(bs.build_basis(name, atoms))
+        assert np.abs(np.diag(S) - 1.0).max() < 1e-13

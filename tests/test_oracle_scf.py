@@ -1,0 +1,89 @@
+"""The SCF oracle (oracle/scf.py) with the host integrals reproduces the
+reference's golden energies at its tolerance approx(abs=5e-8)
+(forte2/helpers/comparisons.py:5)."""
+
+import numpy as np
+import pytest
+
+from forte2_amd.system import System
+from oracle import jk as ojk
+from oracle import scf as oscf
+
+WATER = """O            0.000000000000     0.000000000000    -0.061664597388
+H            0.000000000000    -0.711620616369     0.489330954643
+H            0.000000000000     0.711620616369     0.489330954643"""
+
+
+def _run(xyz, basis, aux, unit="angstrom", ms=None, guess="sap", **kw):
+    s = System(xyz, basis, aux, unit=unit)
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    V_sap = s.sap_potential() if guess == "sap" else None
+    jk = oscf.DFJK(B)
+    if ms is None:
+        return oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, jk, s.Zsum,
+                            X=s.Xorth, V_sap=V_sap, **kw)
+    return oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, jk, s.Zsum, ms,
+                        X=s.Xorth, V_sap=V_sap, **kw)
+
+
+@pytest.mark.parametrize(
+    "xyz,basis,aux,unit,eref",
+    [
+        # tests/orbitals/test_cube.py:15-23
+        (WATER, "cc-pvdz", "cc-pvtz-jkfit", "angstrom", -76.02176598836786),
+        # tests/mcopt/test_casscf_simple.py:5-16
+        ("H 0 0 0\nH 0 0 1.058354421806", "cc-pvdz", "cc-pvtz-jkfit", "angstrom", -1.08928367118043),
+        # tests/mcopt/test_casscf_hf.py:8-20
+        ("H 0.0 0.0 0.0\nF 0.0 0.0 2.0", "cc-pvdz", "cc-pvtz-jkfit", "bohr", -99.9977252002946),
+        # tests/scf/test_rhf.py:12-22
+        (WATER, "cc-pvqz", "cc-pvqz-jkfit", "angstrom", -76.061466407195),
+    ],
+)
+def test_rhf_golden(xyz, basis, aux, unit, eref):
+    r = _run(xyz, basis, aux, unit, e_tol=1e-10)
+    assert r.E == pytest.approx(eref, abs=5e-8)
+
+
+def test_rhf_hcore_guess_same_state():
+    r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", guess="hcore", e_tol=1e-10)
+    assert r.E == pytest.approx(-76.02176598836786, abs=5e-8)
+
+
+def test_uhf_triplet_golden():
+    # tests/scf/test_uhf.py:9-23
+    r = _run(WATER, "cc-pvqz", "cc-pvqz-jkfit", ms=1.0, e_tol=1e-10)
+    assert r.E == pytest.approx(-75.810772399321, abs=5e-8)
+    assert r.S2 == pytest.approx(2.005739321130, abs=5e-8)
+
+
+def test_uhf_singlet_equals_rhf():
+    # tests/scf/test_uhf.py:26-40 (ms = 0 UHF collapses to RHF)
+    r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", ms=0.0, e_tol=1e-10)
+    assert r.E == pytest.approx(-76.02176598836786, abs=5e-8)
+    assert abs(r.S2) < 1e-8
+
+
+def test_scf_failure_paths():
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    with pytest.raises(RuntimeError):
+        oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                     X=s.Xorth, maxiter=2)
+    with pytest.raises(AssertionError):
+        oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - 1,
+                     X=s.Xorth)
+    with pytest.raises(ValueError):
+        oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum, 0.5,
+                     X=s.Xorth)
+
+
+def test_duplicate_aux_shells_make_metric_singular():
+    # jkbuilder.py:141-148 -- a linearly dependent aux basis is rejected
+    from forte2_amd import basis as bs, integrals as ints
+
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit")
+    aux = bs.BasisSet()
+    aux.add(0, [1.0], [1.0], [0, 0, 0])
+    aux.add(0, [1.0], [1.0], [0, 0, 0])
+    with pytest.raises(ValueError):
+        ojk.build_B_density_fitting(ints.coulomb_2c(aux), ints.coulomb_3c(aux, s.basis))
