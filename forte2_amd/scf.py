@@ -1,0 +1,363 @@
+"""RHF / UHF on the device (FP64 linear algebra in HBM, J/K through libf2jk).
+
+Mirrors the reference's SCF driver and its defaults (forte2/scf/scf_base.py:
+62-251; rhf.py:35-65; uhf.py:40-127; helpers/diis.py:10-125):
+  * usage ``RHF(charge=0)(system).run()`` / ``UHF(charge, ms=..)(system).run()``;
+  * DIIS start 1 / nvec 8 / min 2, e_tol 1e-9, d_tol 1e-6 (Frobenius norm of
+    the density change summed over spins), maxiter 100, SAP ("minao") or
+    core-Hamiltonian guess, one extra diagonalisation + Fock build after
+    convergence, RuntimeError on non-convergence;
+  * canonical orthogonalisation X (eigen-truncated at 1e-8), eigh of X^T F X,
+    DIIS and the energy all run on the device in FP64.
+
+Precision form (``precision="bf16x3"`` or ``"fp32"``): the J/K builds of the
+early iterations run in that precision on full densities; when |dE| drops
+below ``switch_inc`` an FP64 reference Fock matrix F_ref(D_ref) is built once
+and later iterations use incremental builds F = F_ref + G_lp(D - D_ref) whose
+error scales with |D - D_ref|; once |dE| < ``switch_fp64`` (1e-6) the
+remaining cycles, including the converged one, use FP64 builds.  The
+difference density of each spin is factored exactly as
+D - D_ref = P A^T + A P^T with the reference orbitals rotated onto the new
+ones (orthogonal Procrustes), P their midpoint and A their difference, so
+K(D - D_ref) = (K[U+] - K[U-]) / 2 with U+- = sqrt(t) P +- A / sqrt(t).
+Every run records, per iteration, the precision used and, at the FP64
+reference build, the error of the low-precision energy against FP64.
+"""
+
+import time
+
+import torch
+
+from .jkbuilder import FockBuilder
+
+
+# ---------------------------------------------------------------- helpers
+def canonical_orth(S, rtol=1e-8):
+    w, U = torch.linalg.eigh(S)
+    if bool((w < -1e-14).any()):
+        raise ValueError("The metric matrix must be positive semi-definite.")
+    w = torch.clamp(w, min=0.0)
+    keep = w >= rtol * w[-1]
+    return U[:, keep] / torch.sqrt(w[keep])
+
+
+class DeviceDIIS:
+    """Pulay CDIIS on device tensors (forte2/helpers/diis.py:67-125)."""
+
+    def __init__(self, diis_start=1, diis_nvec=8, diis_min=2, do_diis=True):
+        if diis_min > diis_nvec:
+            raise ValueError("diis_min must be less than or equal to diis_nvec")
+        self.start, self.nvec, self.min = diis_start, diis_nvec, diis_min
+        self.do = bool(do_diis) and diis_start >= 0 and diis_nvec >= 1
+        self.reset()
+
+    def reset(self):
+        self.p, self.e = [], []
+        self.iter = 0
+        self.status = ""
+
+    def update(self, p, e):
+        self.status = ""
+        if not self.do:
+            return p
+        self.iter += 1
+        if self.iter < self.start:
+            return p
+        self.p.append(p.clone())
+        self.e.append(e.reshape(-1).clone())
+        if len(self.p) > self.nvec:
+            self.p.pop(0)
+            self.e.pop(0)
+        self.status = "S"
+        n = len(self.p)
+        if n < self.min:
+            return p
+        E = torch.stack(self.e)
+        Bm = -torch.ones((n + 1, n + 1), dtype=p.dtype, device=p.device)
+        Bm[-1, -1] = 0.0
+        G = E @ E.T
+        Bm[:n, :n] = G / G.abs().max()
+        rhs = torch.zeros(n + 1, dtype=p.dtype, device=p.device)
+        rhs[-1] = -1.0
+        x, info = torch.linalg.solve_ex(Bm, rhs)
+        if int(info) != 0 or not bool(torch.isfinite(x).all()):
+            return p  # singular DIIS system: skip extrapolation (diis.py:115-120)
+        self.status += "/E"
+        return torch.einsum("i,i...->...", x[:n], torch.stack(self.p))
+
+
+def _procrustes_split(Cn, Co, S):
+    """U+, U- with Cn Cn^T - Co Co^T = (U+ U+^T - U- U-^T) / 2."""
+    if Cn.shape[1] == 0:
+        return None
+    M = Co.T @ S @ Cn
+    U, _, Vh = torch.linalg.svd(M)
+    Coa = Co @ (U @ Vh)
+    P = 0.5 * (Cn + Coa)
+    A = Cn - Coa
+    na, npn = torch.linalg.norm(A), torch.linalg.norm(P)
+    if float(na) == 0.0:
+        return None
+    t = float(na / npn)
+    s = t ** 0.5
+    return s * P + A / s, s * P - A / s
+
+
+# ---------------------------------------------------------------- SCF base
+class _SCF:
+    def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
+                 d_tol=1e-6, maxiter=100, guess_type="minao", die_if_not_converged=True,
+                 precision="fp64", switch_inc=1e-4, switch_fp64=1e-6, incremental=True,
+                 device=None, log=None):
+        self.charge = charge
+        self.do_diis, self.diis_start, self.diis_nvec, self.diis_min = do_diis, diis_start, diis_nvec, diis_min
+        self.e_tol, self.d_tol, self.maxiter = e_tol, d_tol, maxiter
+        self.guess_type = guess_type
+        self.die_if_not_converged = die_if_not_converged
+        if precision not in ("fp64", "fp32", "bf16x3"):
+            raise ValueError(f"unknown precision {precision!r}")
+        self.precision = precision
+        self.switch_inc, self.switch_fp64, self.incremental = switch_inc, switch_fp64, incremental
+        self.device = device
+        self.log = log
+        self.converged = False
+        self.executed = False
+
+    def __call__(self, system):
+        self.system = system
+        self.nel = system.Zsum - self.charge
+        if self.nel < 0:
+            raise AssertionError("Number of electrons must be non-negative.")
+        self._parse_state()
+        self.C = None
+        return self
+
+    # ------------------------------------------------------------------ setup
+    def _setup(self):
+        sysm = self.system
+        fb = sysm.fock_builder
+        if not isinstance(fb, FockBuilder):
+            raise TypeError("system.fock_builder must be a forte2_amd FockBuilder")
+        self.fb = fb
+        dev = fb.device
+        self.dev = dev
+        f64 = torch.float64
+        self.S = torch.as_tensor(sysm.ints_overlap(), dtype=f64).to(dev)
+        self.H = torch.as_tensor(sysm.ints_hcore(), dtype=f64).to(dev)
+        self.Vnn = float(sysm.nuclear_repulsion)
+        self.X = canonical_orth(self.S, getattr(sysm, "overlap_ortho_rtol", 1e-8))
+        self.nbf = sysm.nbf
+        self.nmo = int(self.X.shape[1])
+        self.naux = sysm.naux
+
+    def _eigh(self, F):
+        e, c = torch.linalg.eigh(self.X.T @ F @ self.X)
+        return e, self.X @ c
+
+    def _guess(self):
+        if self.guess_type == "minao":
+            V = torch.as_tensor(self.system.sap_potential(), dtype=torch.float64).to(self.dev)
+            return self._eigh(self.H + V)[1]
+        if self.guess_type == "hcore":
+            return self._eigh(self.H)[1]
+        raise RuntimeError(f"Unknown initial guess type: {self.guess_type}")
+
+    # ------------------------------------------------------------------ precision schedule
+    def _fock_for(self, stage):
+        """Fock matrices for the current orbitals in the given stage."""
+        Cocc = self._occ()
+        if stage in ("fp64", "lp"):
+            prec = "fp64" if stage == "fp64" else self.precision
+            J, K = self.fb.build_JK(Cocc, precision=prec)
+            return self._assemble(J, K)
+        if stage == "ref":
+            J, K = self.fb.build_JK(Cocc, precision="fp64")
+            F = self._assemble(J, K)
+            self._ref = ([c.clone() for c in Cocc], [d.clone() for d in self.D], [f.clone() for f in F])
+            return F
+        # incremental: F = F_ref + G_lp(D - D_ref)
+        Cref, Dref, Fref = self._ref
+        dD = [d - dr for d, dr in zip(self.D, Dref)]
+        Jd = self.fb.build_J([sum(dD)] if len(dD) > 1 else dD, precision=self.precision)[0]
+        Kd = []
+        for cn, co in zip(Cocc, Cref):
+            split = _procrustes_split(cn, co, self.S)
+            if split is None:
+                Kd.append(torch.zeros_like(Jd))
+            else:
+                Kp, Km = self.fb.build_K(list(split), precision=self.precision)
+                Kd.append(0.5 * (Kp - Km))
+        return self._assemble_delta(Fref, Jd, Kd)
+
+    def run(self):
+        t0 = time.monotonic()
+        self._setup()
+        diis = DeviceDIIS(self.diis_start, self.diis_nvec, self.diis_min, self.do_diis)
+        self.iter = 0
+        self.history = []
+        stage = "fp64" if self.precision == "fp64" else "lp"
+        if self.C is None:
+            self.C = self._guess_orbitals()
+        self.D = self._density()
+        F = self._fock_for(stage)
+        self.E = self.Vnn + self._energy(F)
+        Eold, Dold = self.E, self.D
+        self.iter += 1
+        self.history.append((self.iter, stage, self.E))
+        self.lp_error_vs_fp64 = None
+        for _ in range(self.maxiter):
+            grad = self._ao_grad(F)
+            Fx = self._diis_update(diis, F, grad)
+            self.eps, self.C = self._diagonalize(Fx)
+            self.D = self._density()
+            # stage transitions of the precision form (decided on the previous dE)
+            next_stage = stage
+            if stage == "lp" and abs(self._last_dE) < self.switch_inc:
+                next_stage = "ref" if self.incremental else "fp64"
+            elif stage == "inc" and abs(self._last_dE) < self.switch_fp64:
+                next_stage = "fp64"
+            if stage == "lp" and next_stage == "fp64" or stage == "inc" and next_stage == "fp64":
+                diis.reset()
+            F = self._fock_for(next_stage)
+            self.E = self.Vnn + self._energy(F)
+            if next_stage == "ref":
+                # error of the low-precision build against FP64 at the same density
+                Flp = self._fock_for("lp")
+                self.lp_error_vs_fp64 = abs((self.Vnn + self._energy(Flp)) - self.E)
+                diis.reset()
+                next_stage = "inc"
+            stage = next_stage
+            dE = self.E - Eold
+            self._last_dE = dE
+            dD = sum(float(torch.linalg.norm(d - do)) for d, do in zip(self.D, Dold))
+            self.history.append((self.iter + 1, stage, self.E))
+            if self.log:
+                self.log(f"{self.iter + 1:4d} {self.E:20.12f} {dE:12.4e} {dD:12.4e} {stage:>5s} {diis.status}")
+            if stage == "fp64" and abs(dE) < self.e_tol and dD < self.d_tol:
+                self.eps, self.C = self._diagonalize(Fx)
+                self.D = self._density()
+                F = self._fock_for("fp64")
+                self.E = self.Vnn + self._energy(F)
+                self.converged = True
+                break
+            Eold, Dold = self.E, self.D
+            self.iter += 1
+        else:
+            if self.die_if_not_converged:
+                raise RuntimeError(f"{self.method} did not converge in {self.maxiter} iterations.")
+        self.F = F
+        self.S2 = self._spin()
+        self.time = time.monotonic() - t0
+        self.executed = True
+        return self
+
+    _last_dE = float("inf")
+
+
+class RHF(_SCF):
+    """Restricted Hartree-Fock (forte2/scf/rhf.py:13-65)."""
+
+    method = "RHF"
+
+    def _parse_state(self):
+        if self.nel % 2:
+            raise AssertionError("RHF requires an even number of electrons.")
+        self.na = self.nb = self.nel // 2
+        self.ms = 0
+
+    def _guess_orbitals(self):
+        return [self._guess()]
+
+    def _occ(self):
+        return [self.C[0][:, : self.na]]
+
+    def _density(self):
+        c = self.C[0][:, : self.na]
+        return [c @ c.T]
+
+    def _assemble(self, J, K):
+        return [self.H + 2.0 * J[0] - K[0]]
+
+    def _assemble_delta(self, Fref, Jd, Kd):
+        return [Fref[0] + 2.0 * Jd - Kd[0]]
+
+    def _energy(self, F):
+        return float(torch.sum(self.D[0] * (self.H + F[0])))
+
+    def _ao_grad(self, F):
+        f, d = F[0], self.D[0]
+        return self.X.T @ (f @ d @ self.S - self.S @ d @ f) @ self.X
+
+    def _diis_update(self, diis, F, grad):
+        return [diis.update(F[0], grad)]
+
+    def _diagonalize(self, F):
+        e, c = self._eigh(F[0])
+        return [e], [c]
+
+    def _spin(self):
+        return 0.0
+
+
+class UHF(_SCF):
+    """Unrestricted Hartree-Fock (forte2/scf/uhf.py:14-127)."""
+
+    method = "UHF"
+
+    def __init__(self, charge, ms=None, guess_mix=False, **kw):
+        super().__init__(charge, **kw)
+        self.ms = ms
+        self.guess_mix = guess_mix
+
+    def _parse_state(self):
+        if self.ms is None:
+            raise ValueError("Spin projection (ms) must be set for UHF.")
+        tw = int(round(self.ms * 2))
+        if abs(tw - self.ms * 2) > 1e-12:
+            raise AssertionError("ms must be a multiple of 0.5.")
+        if self.nel % 2 != tw % 2:
+            raise ValueError(f"{self.nel} electrons is incompatible with ms={self.ms}!")
+        self.na, self.nb = (self.nel + tw) // 2, (self.nel - tw) // 2
+        if self.na < 0 or self.nb < 0:
+            raise AssertionError("UHF requires non-negative numbers of alpha and beta electrons.")
+
+    def _guess_orbitals(self):
+        C = self._guess()
+        return [C, C.clone()]
+
+    def _occ(self):
+        return [self.C[0][:, : self.na], self.C[1][:, : self.nb]]
+
+    def _density(self):
+        a, b = self._occ()
+        return [a @ a.T, b @ b.T]
+
+    def _assemble(self, J, K):
+        Jt = J[0] + J[1]
+        return [self.H + Jt - k for k in K]
+
+    def _assemble_delta(self, Fref, Jd, Kd):
+        return [f + Jd - k for f, k in zip(Fref, Kd)]
+
+    def _energy(self, F):
+        Da, Db = self.D
+        return float(0.5 * (torch.sum((Da + Db) * self.H) + torch.sum(Da * F[0]) + torch.sum(Db * F[1])))
+
+    def _ao_grad(self, F):
+        return torch.cat([(self.X.T @ (f @ d @ self.S - self.S @ d @ f) @ self.X).reshape(-1)
+                          for d, f in zip(self.D, F)])
+
+    def _diis_update(self, diis, F, grad):
+        N = self.nbf
+        flat = diis.update(torch.cat([f.reshape(-1) for f in F]), grad)
+        return [flat[: N * N].reshape(N, N), flat[N * N:].reshape(N, N)]
+
+    def _diagonalize(self, F):
+        ea, ca = self._eigh(F[0])
+        eb, cb = self._eigh(F[1])
+        return [ea, eb], [ca, cb]
+
+    def _spin(self):
+        a, b = self._occ()
+        Sij = a.T @ self.S @ b
+        return float(self.ms * (self.ms + 1) + self.nb - torch.sum(Sij * Sij))

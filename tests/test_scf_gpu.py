@@ -1,0 +1,112 @@
+"""Device SCF (forte2_amd.scf) against the SCF oracle and the reference's golden values.
+
+Plain form: the device RHF/UHF energy equals the oracle's to <= 1e-10 Eh on the
+same integrals and guess (water; (H2O)_16; an alkane cation).  Precision form:
+the final energy equals the device FP64 energy to <= 1e-8 Eh.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from forte2_amd import geometry as geo
+from forte2_amd.scf import RHF, UHF
+from forte2_amd.system import System
+from oracle import jk as ojk
+from oracle import scf as oscf
+
+pytestmark = pytest.mark.gpu
+
+WATER = """O            0.000000000000     0.000000000000    -0.061664597388
+H            0.000000000000    -0.711620616369     0.489330954643
+H            0.000000000000     0.711620616369     0.489330954643"""
+E_WATER_DZ = -76.02176598836786       # tests/orbitals/test_cube.py:15
+E_WATER_QZ_UHF = -75.810772399321     # tests/scf/test_uhf.py:11
+S2_WATER_QZ_UHF = 2.005739321130
+
+
+def _oracle(s, ms=None, charge=0, e_tol=1e-10):
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    args = (s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - charge)
+    kw = dict(X=s.Xorth, V_sap=s.sap_potential(), e_tol=e_tol)
+    if ms is None:
+        return oscf.run_rhf(*args, **kw), B
+    return oscf.run_uhf(*args, ms, **kw), B
+
+
+def test_device_B_matches_oracle_B(dev):
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    Bd = s.fock_builder.B_Pmn.cpu().numpy()
+    Bo = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    assert np.abs(Bd - Bo).max() < 1e-12
+
+
+def test_rhf_water_plain_form(dev):
+    print("[scf-gpu] water cc-pVDZ/cc-pVTZ-JKFIT RHF", flush=True)
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    scf = RHF(charge=0, e_tol=1e-10)(s).run()
+    ref, _ = _oracle(s)
+    print(f"   device {scf.E:.12f} oracle {ref.E:.12f} diff {scf.E - ref.E:.2e} iters {scf.iter}/{ref.iter}", flush=True)
+    assert abs(scf.E - ref.E) <= 1e-10
+    assert scf.E == pytest.approx(E_WATER_DZ, abs=5e-8)
+    assert scf.converged
+
+
+def test_rhf_h2o16_plain_form(dev):
+    print("[scf-gpu] (H2O)_16 cc-pVDZ/cc-pVTZ-JKFIT RHF", flush=True)
+    s = System(geo.water_cluster(16, 9.0, seed=16), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    assert (s.nbf, s.naux) == (384, 2224)
+    scf = RHF(charge=0, e_tol=1e-10)(s).run()
+    ref, _ = _oracle(s)
+    print(f"   device {scf.E:.12f} oracle {ref.E:.12f} diff {scf.E - ref.E:.2e} iters {scf.iter}/{ref.iter} "
+          f"time {scf.time:.2f}s", flush=True)
+    assert abs(scf.E - ref.E) <= 1e-10
+
+
+def test_uhf_water_triplet_golden(dev):
+    s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit", device=dev)
+    scf = UHF(charge=0, ms=1.0, e_tol=1e-10)(s).run()
+    assert scf.E == pytest.approx(E_WATER_QZ_UHF, abs=5e-8)
+    assert scf.S2 == pytest.approx(S2_WATER_QZ_UHF, abs=5e-8)
+
+
+def test_uhf_alkane_cation_plain_form(dev):
+    print("[scf-gpu] C3H8+ cc-pVDZ/cc-pVTZ-JKFIT UHF", flush=True)
+    s = System(geo.alkane(3), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    scf = UHF(charge=1, ms=0.5, e_tol=1e-10)(s).run()
+    ref, _ = _oracle(s, ms=0.5, charge=1)
+    print(f"   device {scf.E:.12f} oracle {ref.E:.12f} diff {scf.E - ref.E:.2e} S2 {scf.S2:.8f}/{ref.S2:.8f}",
+          flush=True)
+    assert abs(scf.E - ref.E) <= 1e-10
+    assert abs(scf.S2 - ref.S2) <= 1e-7
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("n", [1, 16])
+def test_precision_form_final_energy(dev, precision, n):
+    xyz = WATER if n == 1 else geo.water_cluster(16, 9.0, seed=16)
+    s = System(xyz, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    e64 = RHF(charge=0, e_tol=1e-10)(s).run()
+    mix = RHF(charge=0, e_tol=1e-10, precision=precision)(s).run()
+    stages = [h[1] for h in mix.history]
+    print(f"[scf-gpu] (H2O)_{n} {precision}: E {mix.E:.12f} fp64 {e64.E:.12f} diff {mix.E - e64.E:.2e} "
+          f"stages {''.join(x[0] for x in stages)} lp-vs-fp64 {mix.lp_error_vs_fp64}", flush=True)
+    assert abs(mix.E - e64.E) <= 1e-8
+    assert stages[0] == "lp" and stages[-1] == "fp64"
+    assert mix.lp_error_vs_fp64 is not None
+
+
+def test_generalized_fock_matches_oracle(dev):
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    fb = s.fock_builder
+    B = fb.B_Pmn.cpu().numpy()
+    rng = np.random.default_rng(0)
+    C = rng.standard_normal((s.nbf, 4)) * 0.3
+    A = rng.standard_normal((4, 4))
+    g1 = A @ A.T
+    J, K = fb.build_JK_generalized(C, g1)
+    Jr, Kr = ojk.build_JK_generalized(B, C, g1)
+    assert np.abs(J - Jr).max() < 1e-10 and np.abs(K - Kr).max() < 1e-10
+    H = s.ints_hcore()
+    assert np.abs(fb.build_core_fock(C[:, :2], H) - ojk.build_core_fock(B, C[:, :2], H)).max() < 1e-10
+    assert np.abs(fb.build_active_fock(C, g1) - ojk.build_active_fock(B, C, g1)).max() < 1e-10

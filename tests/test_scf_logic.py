@@ -1,0 +1,70 @@
+"""Host logic of the device SCF, exercised on CPU tensors (no GPU needed)."""
+
+import numpy as np
+import pytest
+import torch
+
+from forte2_amd.scf import DeviceDIIS, _procrustes_split, canonical_orth
+from oracle import scf as oscf
+
+
+def _orthonormal(rng, N, o, S):
+    A = rng.standard_normal((N, o))
+    # S-orthonormalise: C = A (A^T S A)^-1/2
+    w, U = np.linalg.eigh(A.T @ S @ A)
+    return A @ U @ np.diag(w ** -0.5) @ U.T
+
+
+def test_difference_density_factorisation_is_exact():
+    rng = np.random.default_rng(4)
+    N, o = 12, 4
+    R = rng.standard_normal((N, N))
+    S = R @ R.T / N + np.eye(N)
+    Co = _orthonormal(rng, N, o, S)
+    # new orbitals: a small rotation of the old occupied space, S-orthonormalised
+    Cn = Co + 0.05 * rng.standard_normal((N, o))
+    w, U = np.linalg.eigh(Cn.T @ S @ Cn)
+    Cn = Cn @ U @ np.diag(w ** -0.5) @ U.T
+    t = torch.from_numpy
+    Up, Um = _procrustes_split(t(Cn), t(Co), t(S))
+    lhs = 0.5 * (Up @ Up.T - Um @ Um.T)
+    assert float((lhs - t(Cn @ Cn.T - Co @ Co.T)).abs().max()) < 1e-13
+    # balanced factors: both carry a norm ~ sqrt(|P| |A|)
+    assert 0.2 < float(torch.linalg.norm(Up) / torch.linalg.norm(Um)) < 5
+    # a rotation within the occupied space is no change of density
+    Q, _ = np.linalg.qr(rng.standard_normal((o, o)))
+    split = _procrustes_split(t(Co @ Q), t(Co), t(S))
+    if split is not None:
+        assert float((0.5 * (split[0] @ split[0].T - split[1] @ split[1].T)).abs().max()) < 1e-12
+
+
+def test_device_diis_matches_oracle_diis():
+    rng = np.random.default_rng(1)
+    d1, d2 = DeviceDIIS(1, 8, 2), oscf.DIIS(1, 8, 2)
+    for k in range(12):
+        p = rng.standard_normal((5, 5))
+        e = rng.standard_normal((5, 5)) * 0.5 ** k
+        a = d1.update(torch.from_numpy(p), torch.from_numpy(e)).numpy()
+        b = d2.update(p, e)
+        assert np.abs(a - b).max() < 1e-12
+        assert d1.status == d2.status
+
+
+def test_device_diis_skips_singular_system():
+    d = DeviceDIIS(1, 8, 2)
+    p = torch.eye(3, dtype=torch.float64)
+    e = torch.zeros((3, 3), dtype=torch.float64)
+    d.update(p, e)
+    out = d.update(2 * p, e)  # all-zero errors: singular B matrix
+    assert torch.equal(out, 2 * p)
+
+
+def test_canonical_orth_matches_oracle():
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((10, 6))
+    S = A @ A.T + 1e-10 * np.eye(10)  # rank-deficient metric: 4 vectors discarded
+    Xd = canonical_orth(torch.from_numpy(S)).numpy()
+    Xo = oscf.canonical_orth(S.copy())
+    assert Xd.shape == Xo.shape == (10, 6)
+    assert np.abs(Xd.T @ S @ Xd - np.eye(6)).max() < 1e-10
+    assert np.abs(Xd @ Xd.T - Xo @ Xo.T).max() < 1e-8
