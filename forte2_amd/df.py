@@ -11,7 +11,6 @@ With ``df_ortho_rtol`` set, X = M^-1/2 with eigenvalues below rtol*max
 discarded (forte2/helpers/matrix_functions.py:52-92), as in the reference.
 """
 
-import numpy as np
 import torch
 
 from . import integrals as _ints
@@ -35,7 +34,7 @@ def metric_factor(M, df_ortho_rtol=None):
     return "invsqrt", (Uk / torch.sqrt(wk)) @ Uk.T
 
 
-def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=2 << 30):
+def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=1 << 30):
     aux, bas = system.auxiliary_basis, system.basis
     if aux is None:
         raise ValueError("Auxiliary basis is not defined. Define auxiliary_basis_set.")
@@ -59,24 +58,26 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=2 <
         _ints.coulomb_3c(aux, bas, s0, s1, out=blk.numpy())
         B[offs[s0]:offs[s1]].copy_(blk, non_blocking=False)
         s0 = s1
-    # B = L^-1 T (or X T) in place, column block by column block
+    # X = L^-1 (as the reference, jkbuilder.py:150), formed in column blocks so
+    # that no triangular solve has a huge right-hand side
+    if kind == "chol":
+        X = torch.empty_like(fac)
+        eye_blk = 1024
+        for j0 in range(0, Q, eye_blk):
+            j1 = min(Q, j0 + eye_blk)
+            E = torch.zeros((Q, j1 - j0), dtype=torch.float64, device=device)
+            E[j0:j1] = torch.eye(j1 - j0, dtype=torch.float64, device=device)
+            X[:, j0:j1] = torch.linalg.solve_triangular(fac, E, upper=False)
+            del E
+    else:
+        X = fac
+    # B = X T in place, one contiguous column chunk of the (Q, N*N) view at a time
     Bv = B.view(Q, N * N)
     chunk = max(1, min(N * N, solve_bytes // (8 * Q)))
     for c0 in range(0, N * N, chunk):
         c1 = min(N * N, c0 + chunk)
-        if kind == "chol":
-            Bv[:, c0:c1] = torch.linalg.solve_triangular(fac, Bv[:, c0:c1], upper=False)
-        else:
-            Bv[:, c0:c1] = fac @ Bv[:, c0:c1]
+        blk = Bv[:, c0:c1].contiguous()
+        Bv[:, c0:c1] = X @ blk
+        del blk
     return B
 
-
-def build_B_host(system):
-    """Host (numpy) B for small systems: same algebra, used by tools and tests."""
-    M = _ints.coulomb_2c(system.auxiliary_basis)
-    L = np.linalg.cholesky(M)
-    T = _ints.coulomb_3c(system.auxiliary_basis, system.basis)
-    Q, N = T.shape[0], T.shape[1]
-    import scipy.linalg
-
-    return scipy.linalg.solve_triangular(L, T.reshape(Q, N * N), lower=True).reshape(Q, N, N)
