@@ -40,7 +40,51 @@ def parse():
                     help="aux functions in the CPU baseline sample (0 = skip)")
     ap.add_argument("--profile-steps", action="store_true",
                     help="run only the timed mixed steps (for rocprofv3)")
+    ap.add_argument("--scf", action="store_true",
+                    help="instead of the J/K microbenchmark, run a real (H2O)_n RHF in FP64 and in the "
+                         "precision form and report energies, errors and SCF latency")
+    ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
     return ap.parse_args()
+
+
+def scf_mode(args):
+    """Real (H2O)_n cc-pVDZ/cc-pVTZ-JKFIT RHF: integrals on the host streamed into
+    HBM, FP64 plain form and the precision form on the same B."""
+    import torch
+
+    from forte2_amd import geometry as geo
+    from forte2_amd.scf import RHF
+    from forte2_amd.system import System
+
+    dev = torch.device("cuda", 0)
+    n = args.waters
+    box = {16: 9.0, 64: 14.5}.get(n, 3.6 * n ** (1.0 / 3.0))
+    seed = n if args.seed is None else args.seed
+    t0 = time.perf_counter()
+    s = System(geo.water_cluster(n, box, seed=seed), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    fb = s.fock_builder
+    fb.progress = lambda msg: print(f"[scf] {msg}", file=sys.stderr, flush=True)
+    fb.B_Pmn
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    print(f"[scf] (H2O)_{n} N={s.nbf} Q={s.naux}: B formed in {t_b:.1f} s", file=sys.stderr, flush=True)
+    out = {"workload": f"(H2O)_{n} RHF cc-pVDZ/cc-pVTZ-JKFIT, box {box} A, seed {seed}",
+           "nbf": s.nbf, "naux": s.naux, "nocc": s.Zsum // 2, "B_formation_s": t_b, "e_tol": 1e-10}
+    runs = {}
+    for prec in ("fp64", args.mixed):
+        fb.timing = False
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r = RHF(charge=0, e_tol=1e-10, precision=prec)(s).run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        stages = "".join(h[1][0] for h in r.history)
+        runs[prec] = {"E": r.E, "iterations": r.iter, "scf_seconds": dt, "stages": stages,
+                      "lp_error_vs_fp64_Eh": r.lp_error_vs_fp64}
+        print(f"[scf] {prec}: E={r.E:.12f} iters={r.iter} {dt:.2f} s stages={stages}", file=sys.stderr, flush=True)
+    out["runs"] = runs
+    out["final_energy_error_vs_fp64_Eh"] = abs(runs[args.mixed]["E"] - runs["fp64"]["E"])
+    print(json.dumps(out))
 
 
 def timed_builds(fb, C, prec, steps, warmup):
@@ -114,6 +158,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.scf:
+        scf_mode(args)
+        return
     import torch
 
     from forte2_amd.jkbuilder import FockBuilder

@@ -32,6 +32,8 @@ F2JK_SYMBOLS = (
     "f2jk_last_timings",
     "f2jk_last_launches",
     "f2jk_work",
+    "f2jk_group_workspace_size",
+    "f2jk_group_build_jk",
     "f2jk_last_error",
 )
 
@@ -74,6 +76,9 @@ def jk_lib():
     lib.f2jk_last_timings.argtypes = [V, c_double_p, I]
     lib.f2jk_last_launches.argtypes = [V, ctypes.POINTER(I), I]
     lib.f2jk_work.argtypes = [V, I, c_int64_p, I, I, c_double_p, c_double_p]
+    lib.f2jk_group_workspace_size.argtypes = [c_voidp_p, I, ctypes.POINTER(I), c_int64_p, I, I, ctypes.POINTER(sz)]
+    lib.f2jk_group_build_jk.argtypes = [c_voidp_p, I, ctypes.POINTER(I), c_voidp_p, c_int64_p, c_int64_p, c_voidp_p,
+                                        c_voidp_p, I, I, c_voidp_p, ctypes.POINTER(sz), V]
     lib.f2jk_last_error.argtypes = []
     lib.f2jk_last_error.restype = ctypes.c_char_p
     for s in F2JK_SYMBOLS:

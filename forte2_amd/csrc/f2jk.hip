@@ -10,6 +10,7 @@
 
 #include "f2jk.h"
 #include "f2jk_kernels.h"
+#include "f2jk_group.h"
 
 using namespace f2jk;
 
@@ -517,6 +518,41 @@ int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t
     return F2JK_OK;
 }
 
+// --------------------------------------------------------------------------- grouped builds
+int group_plan(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const int64_t* nocc, int precision,
+               int flags, std::vector<Plan>& plans) {
+    if (njobs < 1) return fail(F2JK_EINVAL, "njobs must be >= 1");
+    if (!ctxs || !ndens || !nocc) return fail(F2JK_EINVAL, "null argument");
+    if (precision != F2JK_PREC_FP64) return fail(F2JK_EUNSUPPORTED, "grouped builds are FP64");
+    if ((flags & (F2JK_WANT_J | F2JK_WANT_K)) != (F2JK_WANT_J | F2JK_WANT_K))
+        return fail(F2JK_EUNSUPPORTED, "grouped builds compute J and K");
+    plans.resize(njobs);
+    int64_t k = 0;
+    for (int j = 0; j < njobs; ++j) {
+        if (!ctxs[j]) return fail(F2JK_EINVAL, "null context in group");
+        if (ndens[j] < 1 || ndens[j] > 2) return fail(F2JK_EUNSUPPORTED, "grouped jobs have 1 or 2 densities");
+        int rc = make_plan(ctxs[j], ndens[j], nocc + k, precision, flags, true, plans[j]);
+        if (rc) return rc;
+        const Plan& p = plans[j];
+        if (p.N % 2) return fail(F2JK_EUNSUPPORTED, "grouped builds need an even nbf");
+        if (p.nchunks != 1) return fail(F2JK_EUNSUPPORTED, "grouped builds need X in one chunk");
+        if (ctxs[j]->device != ctxs[0]->device) return fail(F2JK_EINVAL, "group spans devices");
+        k += ndens[j];
+    }
+    return F2JK_OK;
+}
+
+template <int PH>
+int launch_group_phase(const GArgs& a, hipStream_t st) {
+    long long total = 0;
+    for (int j = 0; j < a.njobs; ++j) total += phase_blocks(a.job[j], PH);
+    if (total == 0) return F2JK_OK;
+    const unsigned grid = (unsigned)std::min<long long>(total, 4096);
+    hipLaunchKernelGGL(grouped_kernel<PH>, dim3(grid), dim3(256), 0, st, a);
+    F2JK_LAUNCHED("grouped phase " + std::to_string(PH));
+    return F2JK_OK;
+}
+
 }  // namespace
 
 // =========================================================================== C ABI
@@ -690,6 +726,108 @@ int f2jk_work(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
             }
         *k_flops = f;
     }
+    return F2JK_OK;
+}
+
+int f2jk_group_workspace_size(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const int64_t* nocc,
+                              int precision, int flags, size_t* job_bytes) {
+    if (!job_bytes) return fail(F2JK_EINVAL, "null argument");
+    std::vector<Plan> plans;
+    int rc = group_plan(ctxs, njobs, ndens, nocc, precision, flags, plans);
+    if (rc) return rc;
+    for (int j = 0; j < njobs; ++j) job_bytes[j] = plans[j].total;
+    return F2JK_OK;
+}
+
+int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const double* const* C,
+                        const int64_t* nocc, const int64_t* ldc, double* const* J, double* const* K,
+                        int precision, int flags, void* const* ws, const size_t* ws_bytes, void* stream) {
+    if (!C || !ldc || !J || !K || !ws || !ws_bytes) return fail(F2JK_EINVAL, "null argument");
+    std::vector<Plan> plans;
+    int rc = group_plan(ctxs, njobs, ndens, nocc, precision, flags, plans);
+    if (rc) return rc;
+    F2JK_HIP(hipSetDevice(ctxs[0]->device));
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<GJob> jobs(njobs);
+    int64_t k = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const Plan& p = plans[j];
+        if ((rc = check_ws(p, ws[j], ws_bytes[j]))) return rc;
+        if ((rc = ensure_rowoff(ctxs[j], p.jvec))) return rc;
+        char* w = (char*)ws[j];
+        GJob& g = jobs[j];
+        std::memset(&g, 0, sizeof(g));
+        g.B = (const double*)ctxs[j]->B[F2JK_F64];
+        g.rowoff = ctxs[j]->rowoff[p.jvec];
+        g.Cp = (double*)(w + p.off_cp);
+        g.D = (double*)(w + p.off_d);
+        g.Dpart = (double*)(w + p.off_dpart);
+        g.gpart = (double*)(w + p.off_gpart);
+        g.gamma = (double*)(w + p.off_gamma);
+        g.jpart = (double*)(w + p.off_jpart);
+        g.X = (double*)(w + p.off_x);
+        g.Kpart = (double*)(w + p.off_kpart);
+        g.N = p.N;
+        g.Q = p.Q;
+        g.nd = p.nd;
+        g.OPAD = p.OPAD;
+        for (int d = 0; d < p.nd; ++d, ++k) {
+            if (!C[k] || !J[k] || !K[k]) return fail(F2JK_EINVAL, "null C/J/K pointer in group");
+            if (nocc[k] > 0 && ldc[k] < nocc[k]) return fail(F2JK_EINVAL, "ldc < nocc");
+            g.C[d] = C[k];
+            g.ldc[d] = nocc[k] > 0 ? ldc[k] : 1;
+            g.nocc[d] = (int)nocc[k];
+            g.width[d] = p.width[d];
+            g.ioff[d] = p.ioff[d];
+            g.J[d] = J[k];
+            g.K[d] = K[k];
+        }
+        g.nvec = p.nvec;
+        g.neg = p.neg;
+        g.npc1 = p.npc1;
+        g.npc2 = p.npc2;
+        g.ppw2 = p.ppw2;
+        g.nlower = p.nlower;
+        g.nsplit = std::min(p.nsplit, std::min(p.Pc, p.Q));
+        g.ncolt = p.ncolt;
+        // the D-build tiles use the same lower-tile count as the K tiles
+        if (p.nlower != (int)(cdiv(p.N, kTile) * (cdiv(p.N, kTile) + 1) / 2))
+            return fail(F2JK_EINVAL, "inconsistent tile plan");
+    }
+    f2jk_ctx* tctx = ctxs[0];
+    tim_begin(tctx, flags & F2JK_TIMING);
+    f2jk_ctx* ctx = tctx;  // for TIM
+    TIM(F2JK_PH_TOTAL, true);
+    for (int j0 = 0; j0 < njobs; j0 += kMaxGroup) {
+        GArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.njobs = std::min(kMaxGroup, njobs - j0);
+        for (int j = 0; j < a.njobs; ++j) a.job[j] = jobs[j0 + j];
+        TIM(F2JK_PH_PREP, true);
+        if ((rc = launch_group_phase<GP_PAD>(a, st))) return rc;
+        if ((rc = launch_group_phase<GP_DKS>(a, st))) return rc;
+        if ((rc = launch_group_phase<GP_DRED>(a, st))) return rc;
+        TIM(F2JK_PH_PREP, false);
+        TIM(F2JK_PH_JGAMMA, true);
+        if ((rc = launch_group_phase<GP_GAMMA>(a, st))) return rc;
+        if ((rc = launch_group_phase<GP_GRED>(a, st))) return rc;
+        TIM(F2JK_PH_JGAMMA, false);
+        TIM(F2JK_PH_JACC, true);
+        if ((rc = launch_group_phase<GP_JACC>(a, st))) return rc;
+        if ((rc = launch_group_phase<GP_JRED>(a, st))) return rc;
+        TIM(F2JK_PH_JACC, false);
+        TIM(F2JK_PH_KX, true);
+        if ((rc = launch_group_phase<GP_KX>(a, st))) return rc;
+        TIM(F2JK_PH_KX, false);
+        TIM(F2JK_PH_KS, true);
+        if ((rc = launch_group_phase<GP_KS>(a, st))) return rc;
+        TIM(F2JK_PH_KS, false);
+        TIM(F2JK_PH_KRED, true);
+        if ((rc = launch_group_phase<GP_KRED>(a, st))) return rc;
+        TIM(F2JK_PH_KRED, false);
+    }
+    TIM(F2JK_PH_TOTAL, false);
+    g_last_error.clear();
     return F2JK_OK;
 }
 

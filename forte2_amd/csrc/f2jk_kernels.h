@@ -87,9 +87,9 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 // grid = (n_element_groups, ceil(Q / ppw)), block = 256.
 // ---------------------------------------------------------------------------
 template <typename TB, int VEC, int EPT, int ND>
-__global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
+__device__ __forceinline__ void j_gamma_kernel_body(
     const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
-    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart) {
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart, int bx, int by) {
     __shared__ double red[kJThreads / kWave][kJFlush][ND];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int64_t NN = (int64_t)N * N;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
     double d[EPT][ND][VEC];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-        const int e = blockIdx.x * (kJThreads * EPT) + k * kJThreads + tid;
+        const int e = bx * (kJThreads * EPT) + k * kJThreads + tid;
         off[k] = 0;
 #pragma unroll
         for (int dd = 0; dd < ND; ++dd)
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
         }
     }
 
-    const int P0 = blockIdx.y * ppw;
+    const int P0 = by * ppw;
     const int P1 = min(Q, P0 + ppw);
     for (int Pb = P0; Pb < P1; Pb += kJFlush) {
         const int np = min(kJFlush, P1 - Pb);
@@ -185,11 +185,18 @@ __global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
             double s = 0.0;
 #pragma unroll
             for (int w = 0; w < kJThreads / kWave; ++w) s += red[w][jj][dd];
-            gpart[((int64_t)blockIdx.x * ND + dd) * Q + Pb + jj] = s;
+            gpart[((int64_t)bx * ND + dd) * Q + Pb + jj] = s;
         }
         __syncthreads();
     }
 }
+template <typename TB, int VEC, int EPT, int ND>
+__global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
+    const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart) {
+    j_gamma_kernel_body<TB, VEC, EPT, ND>(B, N, Q, D, ldd, rowoff, nvec, ppw, gpart, blockIdx.x, blockIdx.y);
+}
+
 
 // gamma[dd][P] = sum_g gpart[g][dd][P], g in a fixed order.
 __global__ void j_gamma_reduce_kernel(const double* __restrict__ gpart, int ngroups,
@@ -207,16 +214,16 @@ __global__ void j_gamma_reduce_kernel(const double* __restrict__ gpart, int ngro
 // grid = (n_element_groups, n_p_chunks), block = 256.
 // ---------------------------------------------------------------------------
 template <typename TB, int VEC, int EPT, int ND>
-__global__ __launch_bounds__(kJThreads) void j_accum_kernel(
+__device__ __forceinline__ void j_accum_kernel_body(
     const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
-    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart) {
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart, int bx, int by) {
     const int tid = threadIdx.x;
     const int64_t NN = (int64_t)N * N;
     int64_t off[EPT];
     int eidx[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-        const int e = blockIdx.x * (kJThreads * EPT) + k * kJThreads + tid;
+        const int e = bx * (kJThreads * EPT) + k * kJThreads + tid;
         eidx[k] = e;
         off[k] = 0;
         if (e < nvec) {
@@ -232,7 +239,7 @@ __global__ __launch_bounds__(kJThreads) void j_accum_kernel(
 #pragma unroll
             for (int v = 0; v < VEC; ++v) acc[k][dd][v] = 0.0;
 
-    const int P0 = blockIdx.y * ppw;
+    const int P0 = by * ppw;
     const int P1 = min(Q, P0 + ppw);
     int P = P0;
     for (; P + 1 < P1; P += 2) {
@@ -278,12 +285,19 @@ __global__ __launch_bounds__(kJThreads) void j_accum_kernel(
         if (eidx[k] >= nvec) continue;
 #pragma unroll
         for (int dd = 0; dd < ND; ++dd) {
-            double* dst = jpart + (((int64_t)blockIdx.y * ND + dd) * nvec + eidx[k]) * VEC;
+            double* dst = jpart + (((int64_t)by * ND + dd) * nvec + eidx[k]) * VEC;
 #pragma unroll
             for (int v = 0; v < VEC; ++v) dst[v] = acc[k][dd][v];
         }
     }
 }
+template <typename TB, int VEC, int EPT, int ND>
+__global__ __launch_bounds__(kJThreads) void j_accum_kernel(
+    const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart) {
+    j_accum_kernel_body<TB, VEC, EPT, ND>(B, N, Q, gamma, rowoff, nvec, ppw, jpart, blockIdx.x, blockIdx.y);
+}
+
 
 // J[dd][m][c] = J[dd][c][m] = sum_chunk jpart[chunk][dd][e][v]  (c = col+v <= m)
 template <int VEC, int ND>
@@ -357,10 +371,10 @@ constexpr int kGThreads = 256;
 // VA: elements per global load of A (16 B when the row length allows).
 // ---------------------------------------------------------------------------
 template <typename T, int VA>
-__global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, int64_t R,
+__device__ __forceinline__ void kx_kernel_body(const T* __restrict__ A, int64_t R,
                                                        int K, const T* __restrict__ Cp,
                                                        int OPAD, T* __restrict__ X,
-                                                       int ncolt) {
+                                                       int ncolt, int b, int nblocks) {
     typedef typename Acc<T>::type acc_t;
     constexpr int VB = 16 / sizeof(T);       // B tile loads are always 16 B (OPAD % 16 == 0)
     __shared__ T As[kTile][kBK + 1];
@@ -368,7 +382,7 @@ __global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, 
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = xcd_remap(b, nblocks);
     const int64_t r0 = (int64_t)(t / ncolt) * kTile;
     const int c0 = (t % ncolt) * kTile;
     const int wr = wid >> 1, wc = wid & 1;
@@ -467,6 +481,14 @@ __global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, 
             }
         }
 }
+template <typename T, int VA>
+__global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, int64_t R,
+                                                       int K, const T* __restrict__ Cp,
+                                                       int OPAD, T* __restrict__ X,
+                                                       int ncolt) {
+    kx_kernel_body<T, VA>(A, R, K, Cp, OPAD, X, ncolt, blockIdx.x, gridDim.x);
+}
+
 
 // ---------------------------------------------------------------------------
 // KS:  Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
@@ -474,10 +496,10 @@ __global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, 
 // FP32 inputs accumulate each P in FP32 and fold it into FP64 registers.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, int N, int OPAD,
+__device__ __forceinline__ void ks_kernel_body(const T* __restrict__ X, int N, int OPAD,
                                                        int ioff, int width, int Pc, int nsplit,
                                                        int nlower, double* __restrict__ Kpart,
-                                                       int accumulate) {
+                                                       int accumulate, int b, int nblocks) {
     typedef typename Acc<T>::type acc_t;
     constexpr int VA = 16 / sizeof(T);
     __shared__ T As[kTile][kBK + 1];
@@ -485,7 +507,7 @@ __global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, 
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const int lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int lt = xcd_remap(b, nblocks);
     const int s = lt / nlower;
     int tl = lt % nlower;
     int mt = 0;
@@ -601,6 +623,14 @@ __global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, 
             }
         }
 }
+template <typename T>
+__global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, int N, int OPAD,
+                                                       int ioff, int width, int Pc, int nsplit,
+                                                       int nlower, double* __restrict__ Kpart,
+                                                       int accumulate) {
+    ks_kernel_body<T>(X, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate, blockIdx.x, gridDim.x);
+}
+
 
 // ---------------------------------------------------------------------------
 // BF16x3 precision form.  An FP32 value x is carried as the bf16 pair
@@ -649,9 +679,9 @@ __global__ void pad_ct_bf16_kernel(const double* __restrict__ C, int64_t ldc, in
 // Tile 128 x (32*NBW): one block holds up to 320 occupied columns so that B is
 // read from HBM once.  4 waves as 2 x 2, each 64 x (16*NBW).
 template <int NBW, int VA>
-__global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
+__device__ __forceinline__ void kx_bf16x3_kernel_body(
     const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
-    const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt) {
+    const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt, int b, int nblocks) {
     constexpr int BN = 32 * NBW;
     __shared__ __bf16 Ah[kTile][kPitch16];
     __shared__ __bf16 Al[kTile][kPitch16];
@@ -660,7 +690,7 @@ __global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int t = xcd_remap(b, nblocks);
     const int64_t r0 = (int64_t)(t / ncolt) * kTile;
     const int c0 = (t % ncolt) * BN;
     const int wr = wid >> 1, wc = wid & 1;
@@ -737,21 +767,21 @@ __global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
             ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
             al[i] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
         }
-        // the three products of a block are issued in separate sweeps so that
-        // consecutive MFMAs never depend on each other
+        // Every block is computed, also those past OPAD (their B rows are zero),
+        // so the MFMA sequence has no branches and the accumulators stay in
+        // place; the three products of a block are issued in separate sweeps
+        // so that consecutive MFMAs never depend on each other.
 #pragma unroll
         for (int j = 0; j < NBW; ++j) {
-            if (j < nbv) {
-                const int br = wc * 16 * NBW + j * 16 + (lane & 15);
-                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bh[br][8 * (lane >> 4)]);
-                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bl[br][8 * (lane >> 4)]);
+            const int br = wc * 16 * NBW + j * 16 + (lane & 15);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bh[br][8 * (lane >> 4)]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bl[br][8 * (lane >> 4)]);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
-            }
+            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
         }
         __syncthreads();
     }
@@ -773,13 +803,20 @@ __global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
             }
         }
 }
+template <int NBW, int VA>
+__global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
+    const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
+    const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt) {
+    kx_bf16x3_kernel_body<NBW, VA>(A, R, K, KP, Cth, Ctl, OPAD, Xh, Xl, ncolt, blockIdx.x, gridDim.x);
+}
+
 
 // KS (bf16x3): Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
 // with X given as hi/lo bf16 pairs; width % 32 == 0.  FP32 accumulation over
 // one P, folded into FP64 registers after every P.
-__global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
+__device__ __forceinline__ void ks_bf16x3_kernel_body(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
-    int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate) {
+    int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate, int b, int nblocks) {
     __shared__ __bf16 Ah[kTile][kPitch16];
     __shared__ __bf16 Al[kTile][kPitch16];
     __shared__ __bf16 Bh[kTile][kPitch16];
@@ -787,7 +824,7 @@ __global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const int lt = xcd_remap(blockIdx.x, gridDim.x);
+    const int lt = xcd_remap(b, nblocks);
     const int s = lt / nlower;
     const int tl = lt % nlower;
     int mt = 0;
@@ -848,15 +885,15 @@ __global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
         }
 
     const int nkp = width / kBK16;
-    const int nsteps = (Pb - Pa) * nkp;
-    if (nsteps > 0) load_tiles(Pa, 0);
-    for (int st = 0; st < nsteps; ++st) {
+    if (Pb > Pa) load_tiles(Pa, 0);
+    for (int P = Pa; P < Pb; ++P) {
+    for (int kt = 0; kt < nkp; ++kt) {
         store_tiles();
         __syncthreads();
-        if (st + 1 < nsteps) {
-            const int sn = st + 1;
-            load_tiles(Pa + sn / nkp, (sn % nkp) * kBK16);
-        }
+        if (kt + 1 < nkp)
+            load_tiles(P, (kt + 1) * kBK16);
+        else if (P + 1 < Pb)
+            load_tiles(P + 1, 0);
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -884,15 +921,15 @@ __global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         __syncthreads();
-        if ((st + 1) % nkp == 0) {
+    }
+    // end of one P: fold the FP32 partials into FP64
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < 4; ++j) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
-                    acc[i][j] = fx4{0, 0, 0, 0};
-                }
+            for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
+            acc[i][j] = fx4{0, 0, 0, 0};
         }
     }
     double* Ks = Kpart + (int64_t)s * N * N;
@@ -911,6 +948,12 @@ __global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
             }
         }
 }
+__global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
+    const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
+    int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate) {
+    ks_bf16x3_kernel_body(Xh, Xl, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate, blockIdx.x, gridDim.x);
+}
+
 
 // K[m][n] = K[n][m] = sum_s Kpart[s][m][n] for m >= n (fixed order in s).
 __global__ void k_reduce_kernel(const double* __restrict__ Kpart, int nsplit, int N,

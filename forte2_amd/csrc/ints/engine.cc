@@ -16,6 +16,8 @@
 //     upward recursion are exact to double precision (9.8.2).
 #include <omp.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -249,10 +251,18 @@ struct Shell {
 
 }  // namespace
 
+struct ShellPairCacheEntry;
+
 struct f2ints_basis {
     std::vector<Shell> shells;
     int64_t nbf = 0;
     int maxl = 0;
+    // Hermite expansions of all significant shell pairs (ket form), built once
+    // on the first 3-centre call and reused by every later aux-shell block
+    std::once_flag pairs_once;
+    std::vector<std::pair<int, int>> pair_index;
+    std::vector<void*> pair_data;  // ShellPair*, owned
+    ~f2ints_basis();
 };
 
 namespace {
@@ -513,15 +523,40 @@ int coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* out) {
 }
 
 // ---------------------------------------------------------------- 3-centre Coulomb
+// significant shell pairs (a >= b) of `bs` with their ket-form Hermite data
+const std::vector<std::pair<int, int>>& ket_pairs(const f2ints_basis* cbs) {
+    f2ints_basis* bs = const_cast<f2ints_basis*>(cbs);
+    std::call_once(bs->pairs_once, [bs]() {
+        const int ns = (int)bs->shells.size();
+        std::vector<std::pair<int, int>> all;
+        for (int a = 0; a < ns; ++a)
+            for (int b = 0; b <= a; ++b) all.push_back({a, b});
+        std::vector<ShellPair*> built(all.size(), nullptr);
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
+        for (size_t q = 0; q < all.size(); ++q) {
+            ShellPair* sp = new ShellPair();
+            make_pair(bs->shells[all[q].first], bs->shells[all[q].second], true, *sp);
+            if (sp->prims.empty()) {
+                delete sp;
+                sp = nullptr;
+            }
+            built[q] = sp;
+        }
+        for (size_t q = 0; q < all.size(); ++q)
+            if (built[q]) {
+                bs->pair_index.push_back(all[q]);
+                bs->pair_data.push_back(built[q]);
+            }
+    });
+    return bs->pair_index;
+}
+
 int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1, double* out) {
     const Harmonics& Hm = harm();
     const HermIndex& H = hidx();
-    const int nsb = (int)bs->shells.size();
     const int64_t nbf = bs->nbf;
     const int64_t first = aux->shells[sh0].off;
-    std::vector<std::pair<int, int>> pairs;
-    for (int a = 0; a < nsb; ++a)
-        for (int b = 0; b <= a; ++b) pairs.push_back({a, b});
+    const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
     const int64_t nP = (sh1 < (int)aux->shells.size() ? aux->shells[sh1].off : aux->nbf) - first;
     std::fill(out, out + (size_t)nP * nbf * nbf, 0.0);
     int maxlaux = 0;
@@ -532,13 +567,11 @@ int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1
         std::vector<double> R(nherm(Lmax)), work((size_t)(Lmax + 1) * nherm(Lmax));
         std::vector<double> Y((size_t)nsph(maxlaux) * nherm(2 * bs->maxl));
         std::vector<double> acc;
-        ShellPair sp;
 #pragma omp for schedule(dynamic)
         for (size_t q = 0; q < pairs.size(); ++q) {
             const Shell& A = bs->shells[pairs[q].first];
             const Shell& B = bs->shells[pairs[q].second];
-            make_pair(A, B, true, sp);
-            if (sp.prims.empty()) continue;
+            const ShellPair& sp = *static_cast<const ShellPair*>(bs->pair_data[q]);
             const int lab = A.l + B.l, nhab = nherm(lab), nsa = nsph(A.l), nsbb = nsph(B.l);
             const int nab = nsa * nsbb;
             for (int s = sh0; s < sh1; ++s) {
@@ -654,6 +687,10 @@ int coulomb_4c(const f2ints_basis* bs, double* out) {
 }
 
 }  // namespace
+
+f2ints_basis::~f2ints_basis() {
+    for (void* p : pair_data) delete static_cast<ShellPair*>(p);
+}
 
 // =========================================================================== C ABI
 extern "C" {

@@ -34,7 +34,23 @@ def metric_factor(M, df_ortho_rtol=None):
     return "invsqrt", (Uk / torch.sqrt(wk)) @ Uk.T
 
 
-def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=1 << 30):
+def _fitting_matrix(kind, fac, blk=1024):
+    """X with B = X T: L^-1 (as the reference, jkbuilder.py:150), formed in
+    column blocks so that no triangular solve has a huge right-hand side."""
+    if kind != "chol":
+        return fac
+    Q = fac.shape[0]
+    X = torch.empty_like(fac)
+    for j0 in range(0, Q, blk):
+        j1 = min(Q, j0 + blk)
+        E = torch.zeros((Q, j1 - j0), dtype=fac.dtype, device=fac.device)
+        E[j0:j1] = torch.eye(j1 - j0, dtype=fac.dtype, device=fac.device)
+        X[:, j0:j1] = torch.linalg.solve_triangular(fac, E, upper=False)
+    return X
+
+
+def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=1 << 30, progress=None):
+    """B (naux, nbf, nbf) FP64 on `device`; `progress(msg)` is called per host block."""
     aux, bas = system.auxiliary_basis, system.basis
     if aux is None:
         raise ValueError("Auxiliary basis is not defined. Define auxiliary_basis_set.")
@@ -57,20 +73,10 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=1 <
         blk = host[:nP]
         _ints.coulomb_3c(aux, bas, s0, s1, out=blk.numpy())
         B[offs[s0]:offs[s1]].copy_(blk, non_blocking=False)
+        if progress is not None:
+            progress(f"(Q|mn) aux functions {offs[s1]}/{Q} on device")
         s0 = s1
-    # X = L^-1 (as the reference, jkbuilder.py:150), formed in column blocks so
-    # that no triangular solve has a huge right-hand side
-    if kind == "chol":
-        X = torch.empty_like(fac)
-        eye_blk = 1024
-        for j0 in range(0, Q, eye_blk):
-            j1 = min(Q, j0 + eye_blk)
-            E = torch.zeros((Q, j1 - j0), dtype=torch.float64, device=device)
-            E[j0:j1] = torch.eye(j1 - j0, dtype=torch.float64, device=device)
-            X[:, j0:j1] = torch.linalg.solve_triangular(fac, E, upper=False)
-            del E
-    else:
-        X = fac
+    X = _fitting_matrix(kind, fac)
     # B = X T in place, one contiguous column chunk of the (Q, N*N) view at a time
     Bv = B.view(Q, N * N)
     chunk = max(1, min(N * N, solve_bytes // (8 * Q)))
@@ -81,3 +87,23 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=1 <
         del blk
     return B
 
+
+
+def host_df_integrals(system):
+    """(M, T) on the host: the Coulomb metric and all of (Q|mn) (small jobs)."""
+    return (_ints.coulomb_2c(system.auxiliary_basis),
+            _ints.coulomb_3c(system.auxiliary_basis, system.basis))
+
+
+def device_B_from_host(M, T, device, df_ortho_rtol=None, solve_bytes=1 << 30):
+    """B = L^-1 T on the device from host (M, T) (same algebra as build_B_density_fitting)."""
+    Q, N = T.shape[0], T.shape[1]
+    kind, fac = metric_factor(torch.from_numpy(M).to(device), df_ortho_rtol)
+    B = torch.from_numpy(T).to(device)
+    X = _fitting_matrix(kind, fac)
+    Bv = B.view(Q, N * N)
+    chunk = max(1, min(N * N, solve_bytes // (8 * Q)))
+    for c0 in range(0, N * N, chunk):
+        c1 = min(N * N, c0 + chunk)
+        Bv[:, c0:c1] = X @ Bv[:, c0:c1].contiguous()
+    return B

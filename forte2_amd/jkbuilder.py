@@ -61,6 +61,7 @@ class FockBuilder:
         self.timing = False
         self.last_timings = None
         self.last_launches = None
+        self.progress = None
         if precision not in nat.PREC:
             raise ValueError(f"unknown precision {precision!r}")
         self.precision = precision
@@ -90,7 +91,7 @@ class FockBuilder:
         if self._B64 is None:
             from .df import build_B_density_fitting
 
-            self._set_B(build_B_density_fitting(self.system, device=self.device))
+            self._set_B(build_B_density_fitting(self.system, device=self.device, progress=self.progress))
         return self._B64
 
     def B_fp32(self):

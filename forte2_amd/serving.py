@@ -1,0 +1,370 @@
+"""Continuous-batching request layer for streams of SCF jobs on one GPU.
+
+A request is an RHF (or UHF) job: a geometry, basis sets and charge/spin.  The
+server keeps each admitted job's B, density, Fock and DIIS history on the
+device.  It drives all active jobs in lock-step ticks.  At each tick, every
+job that needs a Fock matrix contributes its occupied orbitals to one grouped
+J/K build (``f2jk_group_build_jk``: one persistent launch per phase for up to
+12 jobs).  Then every job advances one SCF step on the device.
+
+Admission is continuous:
+  * a request becomes eligible at its arrival time;
+  * it is admitted, in arrival order, as soon as a slot is free and its HBM
+    footprint (B = naux*nbf^2*8 bytes, plus the J/K workspace) fits the
+    budget left by the jobs in flight;
+  * its one-time host work (integrals, (Q|mn), SAP guess potential) runs in a
+    worker thread pool while the GPU serves the other jobs;
+  * its B is formed on the device when it joins the next tick.
+
+A job's J/K do not depend on what else is in the group (the grouped kernels
+reuse the single-job decomposition), so a result is bitwise the same alone
+and in any mix.
+
+Per request the server records arrival, admission, first tick and completion
+times.  latency = completion - arrival.  ``summary()`` reports p50/p99
+latency and jobs/s.
+"""
+
+import concurrent.futures as cf
+import ctypes
+import dataclasses
+import time
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import df as _df
+from .jkbuilder import FockBuilder
+from .scf import RHF, UHF, DeviceDIIS
+from .system import System
+
+MAX_GROUP = 12
+
+
+@dataclasses.dataclass
+class SCFRequest:
+    id: int
+    xyz: str
+    basis: str = "cc-pvdz"
+    aux: str = "cc-pvtz-jkfit"
+    charge: int = 0
+    ms: float = None           # None -> RHF, else UHF with this ms
+    arrival: float = 0.0       # seconds after the start of the run
+
+
+@dataclasses.dataclass
+class SCFResult:
+    id: int
+    E: float = None
+    iterations: int = 0
+    converged: bool = False
+    nbf: int = 0
+    naux: int = 0
+    arrival: float = 0.0
+    admitted: float = 0.0
+    started: float = 0.0
+    finished: float = 0.0
+    error: str = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def job_footprint(nbf, naux, nocc_total):
+    """HBM bytes a job holds: B plus an upper bound on its J/K workspace."""
+    N, Q = nbf, naux
+    opad = 32 * (int(nocc_total) // 16 + 2)
+    B = 8 * Q * N * N
+    ws = 8 * Q * N * opad + 8 * N * N * (4 + 80) + 8 * 2048 * 4 * 1024 + (64 << 20)
+    return B + ws
+
+
+class _Job:
+    """One SCF run as a state machine, driven from outside one J/K build at a time."""
+
+    def __init__(self, req, system, scf, fb):
+        self.req, self.system, self.scf, self.fb = req, system, scf, fb
+        self.phase = "init"
+        self.Fx = None
+
+    def occ(self):
+        return self.scf._occ()
+
+    def start(self, V_sap):
+        s = self.scf
+        s._setup()
+        s.C = s._guess_orbitals() if V_sap is None else self._sap_guess(V_sap)
+        s.D = s._density()
+        s.iter = 0
+        self.diis = DeviceDIIS(s.diis_start, s.diis_nvec, s.diis_min, s.do_diis)
+
+    def _sap_guess(self, V_sap):
+        s = self.scf
+        C = s._eigh(s.H + torch.as_tensor(V_sap, dtype=torch.float64).to(s.dev))[1]
+        return [C] if isinstance(s, RHF) else [C, C.clone()]
+
+    def _prepare_next(self, F):
+        s = self.scf
+        grad = s._ao_grad(F)
+        self.Fx = s._diis_update(self.diis, F, grad)
+        s.eps, s.C = s._diagonalize(self.Fx)
+        s.D = s._density()
+
+    def advance(self, J, K):
+        """Consume this job's J/K; returns True when the job is finished."""
+        s = self.scf
+        F = s._assemble(J, K)
+        s.E = s.Vnn + s._energy(F)
+        if self.phase == "init":
+            self.Eold, self.Dold = s.E, s.D
+            s.iter = 1
+            self.phase = "loop"
+            self._prepare_next(F)
+            return False
+        if self.phase == "final":
+            s.converged = True
+            s.F = F
+            return True
+        dE = s.E - self.Eold
+        dD = sum(float(torch.linalg.norm(d - do)) for d, do in zip(s.D, self.Dold))
+        if abs(dE) < s.e_tol and dD < s.d_tol:
+            s.eps, s.C = s._diagonalize(self.Fx)
+            s.D = s._density()
+            self.phase = "final"
+            return False
+        self.Eold, self.Dold = s.E, s.D
+        s.iter += 1
+        if s.iter > s.maxiter:
+            raise RuntimeError(f"{s.method} did not converge in {s.maxiter} iterations.")
+        self._prepare_next(F)
+        return False
+
+
+class SCFServer:
+    """Serve a stream of SCF requests on one GPU with continuous batching.
+
+    Parameters
+    ----------
+    device : torch device (default cuda:0)
+    hbm_budget : bytes of HBM the jobs in flight may hold (default: 85% of free)
+    max_active : jobs in flight at once (slots)
+    prep_threads : host threads computing the one-time integrals of admitted jobs
+    grouped : batch the J/K builds of a tick into grouped launches
+    e_tol, d_tol, maxiter : SCF convergence (forte2 semantics)
+    """
+
+    def __init__(self, device=None, hbm_budget=None, max_active=12, prep_threads=4, grouped=True,
+                 e_tol=1e-8, d_tol=1e-6, maxiter=100, log=None):
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(0.85 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.max_active = int(max_active)
+        self.prep_threads = int(prep_threads)
+        self.grouped = grouped
+        self.e_tol, self.d_tol, self.maxiter = e_tol, d_tol, maxiter
+        self.log = log
+        self.results = {}
+        self.ticks = 0
+        self.group_sizes = []
+
+    # ------------------------------------------------------------------ host preparation
+    @staticmethod
+    def _prepare_host(req):
+        """Host-only work of a request (runs in a worker thread)."""
+        system = System(req.xyz, req.basis, req.aux)
+        system.ints_hcore()  # cached on the system: no host integrals on the serving thread
+        M, T = _df.host_df_integrals(system)
+        V_sap = system.sap_potential()
+        return system, M, T, V_sap
+
+    def _activate(self, req, prepared):
+        system, M, T, V_sap = prepared
+        system.device = self.device
+        B = _df.device_B_from_host(M, T, self.device)
+        fb = FockBuilder(system, B=B, device=self.device)
+        system.fock_builder = fb
+        kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter)
+        scf = RHF(req.charge, **kw) if req.ms is None else UHF(req.charge, ms=req.ms, **kw)
+        scf(system)
+        job = _Job(req, system, scf, fb)
+        job.start(V_sap)
+        return job
+
+    # ------------------------------------------------------------------ grouped J/K
+    def _build_group(self, jobs):
+        """J/K for every job in `jobs` (same density count); returns [(J, K)] per job."""
+        lib = nat.jk_lib()
+        ctxs, nds, Cs, nocc, ldc, Js, Ks = [], [], [], [], [], [], []
+        outs = []
+        for jb in jobs:
+            occ = [c.contiguous() for c in jb.occ()]
+            N = jb.fb.nbf
+            J = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ]
+            K = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ]
+            jb._keep = occ
+            ctxs.append(jb.fb.ctx.value)
+            nds.append(len(occ))
+            for c, j, k in zip(occ, J, K):
+                Cs.append(c.data_ptr())
+                nocc.append(c.shape[1])
+                ldc.append(max(1, c.stride(0)))
+                Js.append(j.data_ptr())
+                Ks.append(k.data_ptr())
+            outs.append((J, K))
+        n = len(jobs)
+        ctx_a = nat.ptr_array(ctxs)
+        nd_a = (ctypes.c_int * n)(*nds)
+        nocc_a = nat.i64_array(nocc)
+        sizes = (ctypes.c_size_t * n)()
+        rc = lib.f2jk_group_workspace_size(ctx_a, n, nd_a, nocc_a, nat.PREC["fp64"],
+                                           nat.F2JK_WANT_J | nat.F2JK_WANT_K, sizes)
+        nat.jk_check(rc, "f2jk_group_workspace_size")
+        ws = [jb.fb._workspace(sizes[i]) for i, jb in enumerate(jobs)]
+        ws_a = nat.ptr_array([w.data_ptr() for w in ws])
+        wsb = (ctypes.c_size_t * n)(*[w.numel() for w in ws])
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        rc = lib.f2jk_group_build_jk(ctx_a, n, nd_a, nat.ptr_array(Cs), nocc_a, nat.i64_array(ldc),
+                                     nat.ptr_array(Js), nat.ptr_array(Ks), nat.PREC["fp64"],
+                                     nat.F2JK_WANT_J | nat.F2JK_WANT_K, ws_a, wsb, stream)
+        nat.jk_check(rc, "f2jk_group_build_jk")
+        return outs
+
+    def _tick(self, active):
+        """One J/K build for every active job, then one SCF step each."""
+        done = []
+        by_nd = {}
+        for jb in active:
+            by_nd.setdefault(len(jb.occ()), []).append(jb)
+        for nd, jobs in by_nd.items():
+            for g0 in range(0, len(jobs), MAX_GROUP):
+                grp = jobs[g0:g0 + MAX_GROUP]
+                if self.grouped:
+                    outs = self._build_group(grp)
+                else:
+                    outs = [jb.fb.build_JK(jb.occ()) for jb in grp]
+                self.group_sizes.append(len(grp))
+                for jb, (J, K) in zip(grp, outs):
+                    try:
+                        if jb.advance(J, K):
+                            done.append(jb)
+                    except RuntimeError as e:
+                        jb.error = str(e)
+                        done.append(jb)
+        self.ticks += 1
+        return done
+
+    # ------------------------------------------------------------------ main loop
+    def run(self, requests):
+        """Serve `requests` (arrival times in seconds from now); returns {id: SCFResult}."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.id))
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        queue, preparing, active = [], {}, []
+        used = 0
+        pool = cf.ThreadPoolExecutor(max_workers=self.prep_threads)
+        pi = 0
+        try:
+            while pi < len(pending) or queue or preparing or active:
+                t = now()
+                while pi < len(pending) and pending[pi].arrival <= t:
+                    queue.append(pending[pi])
+                    pi += 1
+                # admission in arrival order: slot and HBM budget
+                while queue and len(active) + len(preparing) < self.max_active:
+                    req = queue[0]
+                    fp = self._estimate(req)
+                    if fp > self.hbm_budget:  # can never fit: rejected, the stream goes on
+                        queue.pop(0)
+                        self.results[req.id] = SCFResult(req.id, arrival=req.arrival, admitted=now(),
+                                                         finished=now(),
+                                                         error=f"footprint {fp} B exceeds the HBM budget")
+                        continue
+                    if used + fp > self.hbm_budget:
+                        break
+                    queue.pop(0)
+                    used += fp
+                    res = SCFResult(req.id, arrival=req.arrival, admitted=now())
+                    self.results[req.id] = res
+                    preparing[pool.submit(self._prepare_host, req)] = (req, fp)
+                # jobs whose host work is done join the next tick
+                for fut in [f for f in preparing if f.done()]:
+                    req, fp = preparing.pop(fut)
+                    res = self.results[req.id]
+                    try:
+                        job = self._activate(req, fut.result())
+                        job.footprint = fp
+                        job.error = None
+                        res.started = now()
+                        res.nbf, res.naux = job.fb.nbf, job.fb.naux
+                        active.append(job)
+                    except Exception as e:  # failed job: others unaffected
+                        res.error = repr(e)
+                        res.finished = now()
+                        used -= fp
+                if active:
+                    for jb in self._tick(active):
+                        active.remove(jb)
+                        res = self.results[jb.req.id]
+                        res.E, res.iterations = jb.scf.E, jb.scf.iter
+                        res.converged, res.error = bool(jb.scf.converged), jb.error
+                        res.finished = now()
+                        used -= jb.footprint
+                        jb.fb._ws = None
+                        del jb
+                    if self.log:
+                        self.log(f"t={now():7.2f}s active={len(active)} preparing={len(preparing)} "
+                                 f"queued={len(queue)} done={sum(1 for r in self.results.values() if r.finished)}")
+                elif preparing:
+                    cf.wait(list(preparing), timeout=0.05, return_when=cf.FIRST_COMPLETED)
+                elif pi < len(pending):
+                    time.sleep(max(0.0, min(0.05, pending[pi].arrival - now())))
+        finally:
+            pool.shutdown(wait=True)
+        self.wall = now()
+        return self.results
+
+    def _estimate(self, req):
+        from . import basis as _basis
+        from . import geometry as _geom
+
+        atoms = _geom.parse_geometry(req.xyz)
+        nbf = _basis.build_basis(req.basis, atoms).size
+        naux = _basis.build_basis(req.aux, atoms).size
+        nel = sum(z for z, _ in atoms) - req.charge
+        return job_footprint(nbf, naux, nel)
+
+    def summary(self):
+        ok = [r for r in self.results.values() if r.error is None and r.converged]
+        lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+        return {
+            "requests": len(self.results),
+            "completed": len(ok),
+            "failed": len(self.results) - len(ok),
+            "wall_s": self.wall,
+            "jobs_per_s": len(ok) / self.wall if self.wall > 0 else 0.0,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_mean_s": float(lat.mean()),
+            "ticks": self.ticks,
+            "mean_group_size": float(np.mean(self.group_sizes)) if self.group_sizes else 0.0,
+        }
+
+
+def serving_stream(nreq=512, rate=4.0, nmax=24, seed=7, basis="cc-pvdz", aux="cc-pvtz-jkfit"):
+    """The serving workload: (H2O)_n with n ~ U{1..nmax}, seeded geometries in a
+    cube of side 3.6 n^(1/3) A, Poisson arrivals at `rate` per second."""
+    from . import geometry as _geom
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(nreq):
+        n = int(rng.integers(1, nmax + 1))
+        xyz = _geom.water_cluster(n, seed=int(rng.integers(0, 2 ** 31)))
+        reqs.append(SCFRequest(i, xyz, basis, aux, arrival=t))
+        t += float(rng.exponential(1.0 / rate))
+    return reqs

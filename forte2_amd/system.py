@@ -43,6 +43,7 @@ class System:
         self.overlap_ortho_rtol = overlap_ortho_rtol
         self.nuclear_repulsion = _ints.nuclear_repulsion(self.atoms)
         self._S = None
+        self._H = None
         self.Xorth, self.nmo = canonical_orth(self.ints_overlap(), overlap_ortho_rtol)
         self.two_component = False
         self.device = device
@@ -61,7 +62,9 @@ class System:
         return _ints.nuclear(self.basis, self.atoms)
 
     def ints_hcore(self):
-        return self.ints_kinetic() + self.ints_nuclear()
+        if self._H is None:
+            self._H = self.ints_kinetic() + self.ints_nuclear()
+        return self._H.copy()
 
     def get_Xorth(self):
         return self.Xorth

@@ -132,6 +132,22 @@ int f2jk_last_launches(const f2jk_ctx* ctx, int* counts, int n);
 int f2jk_work(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision,
               int flags, double* j_bytes, double* k_flops);
 
+/* Grouped builds for the serving layer (no counterpart in the reference,
+ * which builds one job at a time): build_JK for njobs independent jobs, each
+ * with its own context (B), 1 or 2 densities and its own workspace, in one
+ * persistent launch per phase.  Arrays C, nocc, ldc, J, K are flattened over
+ * jobs (sum of ndens entries); ws/ws_bytes have one entry per job, sized by
+ * f2jk_group_workspace_size.  A job's results are bitwise those of
+ * f2jk_build_jk on the same inputs, whatever else is in the group.
+ * FP64 only; nbf even; the job's X must fit one chunk (EUNSUPPORTED otherwise). */
+int f2jk_group_workspace_size(f2jk_ctx* const* ctxs, int njobs, const int* ndens,
+                              const int64_t* nocc, int precision, int flags,
+                              size_t* job_bytes);
+int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens,
+                        const double* const* C, const int64_t* nocc, const int64_t* ldc,
+                        double* const* J, double* const* K, int precision, int flags,
+                        void* const* ws, const size_t* ws_bytes, void* stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* f2jk_last_error(void);
 
