@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--waters", type=int, default=64, help="(H2O)_n cluster size")
-    ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--mixed", default="bf16x3", choices=["fp32", "bf16x3"])
     ap.add_argument("--cpu-sample-aux", type=int, default=384,
                     help="aux functions in the CPU baseline sample (0 = skip)")
     ap.add_argument("--profile-steps", action="store_true",
@@ -44,7 +44,23 @@ def parse():
                     help="instead of the J/K microbenchmark, run a real (H2O)_n RHF in FP64 and in the "
                          "precision form and report energies, errors and SCF latency")
     ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 4 GiB device copy (x3), the reference for HBM byte counters")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of 4 GiB, three times: per launch it reads
+    and writes exactly 4 GiB, which calibrates FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = (4 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": 4 << 30, "launches": 3}))
 
 
 def scf_mode(args):
@@ -160,6 +176,9 @@ def main():
         sys.exit(2)
     if args.scf:
         scf_mode(args)
+        return
+    if args.copy_calibration:
+        copy_calibration()
         return
     import torch
 

@@ -57,6 +57,7 @@ struct f2jk_ctx {
     int* rowoff[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // by VEC (1, 2, 4)
     int nvec[5] = {0, 0, 0, 0, 0};
     size_t xchunk_bytes = kDefaultXChunkBytes;
+    bool j_nontemporal = false;
     int target_blocks = kTargetBlocks;
     // timing
     hipEvent_t ev[kMaxEvents];      // (start, end) pairs of non-nested phases
@@ -243,7 +244,7 @@ int tim_mark(f2jk_ctx* ctx, int phase, hipStream_t st, bool start) {
     } while (0)
 
 // --------------------------------------------------------------------------- J
-template <typename TB, int VEC, int ND>
+template <typename TB, int VEC, int ND, bool NT>
 int launch_j(f2jk_ctx* ctx, const Plan& p, const TB* B, const double* const* D, int64_t ldd,
              double* const* J, char* ws, hipStream_t st) {
     DPtrs<ND> dp;
@@ -257,7 +258,7 @@ int launch_j(f2jk_ctx* ctx, const Plan& p, const TB* B, const double* const* D, 
     double* jpart = (double*)(ws + p.off_jpart);
     const int* rowoff = ctx->rowoff[VEC];
     TIM(F2JK_PH_JGAMMA, true);
-    hipLaunchKernelGGL((j_gamma_kernel<TB, VEC, kEPT, ND>), dim3(p.neg, p.npc1), dim3(kJThreads), 0,
+    hipLaunchKernelGGL((j_gamma_kernel<TB, VEC, kEPT, ND, NT>), dim3(p.neg, p.npc1), dim3(kJThreads), 0,
                        st, B, p.N, p.Q, dp, ldd, rowoff, p.nvec, kJ1PerBlock, gpart);
     F2JK_LAUNCHED("j_gamma");
     const int64_t ng = (int64_t)ND * p.Q;
@@ -266,7 +267,7 @@ int launch_j(f2jk_ctx* ctx, const Plan& p, const TB* B, const double* const* D, 
     F2JK_LAUNCHED("j_gamma_reduce");
     TIM(F2JK_PH_JGAMMA, false);
     TIM(F2JK_PH_JACC, true);
-    hipLaunchKernelGGL((j_accum_kernel<TB, VEC, kEPT, ND>), dim3(p.neg, p.npc2), dim3(kJThreads), 0,
+    hipLaunchKernelGGL((j_accum_kernel<TB, VEC, kEPT, ND, NT>), dim3(p.neg, p.npc2), dim3(kJThreads), 0,
                        st, B, p.N, p.Q, gamma, rowoff, p.nvec, p.ppw2, jpart);
     F2JK_LAUNCHED("j_accum");
     hipLaunchKernelGGL((j_reduce_kernel<VEC, ND>), dim3((unsigned)cdiv(p.nvec, 256)), dim3(256), 0,
@@ -283,10 +284,16 @@ int launch_j_nd(f2jk_ctx* ctx, const Plan& p, const TB* B, int nd, const double*
     int d = 0;
     while (d < nd) {
         int rc;
-        if (nd - d >= 2)
-            rc = launch_j<TB, VEC, 2>(ctx, p, B, D + d, ldd, J + d, ws, st), d += 2;
-        else
-            rc = launch_j<TB, VEC, 1>(ctx, p, B, D + d, ldd, J + d, ws, st), d += 1;
+        const bool nt = ctx->j_nontemporal;
+        if (nd - d >= 2) {
+            rc = nt ? launch_j<TB, VEC, 2, true>(ctx, p, B, D + d, ldd, J + d, ws, st)
+                    : launch_j<TB, VEC, 2, false>(ctx, p, B, D + d, ldd, J + d, ws, st);
+            d += 2;
+        } else {
+            rc = nt ? launch_j<TB, VEC, 1, true>(ctx, p, B, D + d, ldd, J + d, ws, st)
+                    : launch_j<TB, VEC, 1, false>(ctx, p, B, D + d, ldd, J + d, ws, st);
+            d += 1;
+        }
         if (rc) return rc;
     }
     return F2JK_OK;
@@ -632,6 +639,8 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
     if (!strcmp(key, "xchunk_bytes")) {
         if (value < (1 << 20)) return fail(F2JK_EINVAL, "xchunk_bytes must be >= 1 MiB");
         ctx->xchunk_bytes = (size_t)value;
+    } else if (!strcmp(key, "j_nontemporal")) {
+        ctx->j_nontemporal = value != 0;
     } else if (!strcmp(key, "target_blocks")) {
         if (value < 1 || value > (1 << 20)) return fail(F2JK_EINVAL, "target_blocks out of range");
         ctx->target_blocks = (int)value;

@@ -44,6 +44,30 @@ __device__ __forceinline__ Vec<T, V> load_vec(const T* p) {
     return *reinterpret_cast<const Vec<T, V>*>(p);
 }
 
+template <typename T, int V>
+struct NTVec {
+    typedef T type __attribute__((ext_vector_type(V)));
+};
+
+// streaming load that bypasses cache retention (data read once per pass)
+template <typename T, int V, bool NT>
+__device__ __forceinline__ Vec<T, V> stream_vec(const T* p) {
+    if constexpr (NT && V > 1) {
+        typedef typename NTVec<T, V>::type vt;
+        const vt x = __builtin_nontemporal_load(reinterpret_cast<const vt*>(p));
+        Vec<T, V> r;
+#pragma unroll
+        for (int v = 0; v < V; ++v) r.v[v] = x[v];
+        return r;
+    } else if constexpr (NT) {
+        Vec<T, V> r;
+        r.v[0] = __builtin_nontemporal_load(p);
+        return r;
+    } else {
+        return load_vec<T, V>(p);
+    }
+}
+
 template <int ND>
 struct DPtrs {
     const double* p[ND];
@@ -86,7 +110,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 // J pass 1: partial gamma per (element group, density, P).
 // grid = (n_element_groups, ceil(Q / ppw)), block = 256.
 // ---------------------------------------------------------------------------
-template <typename TB, int VEC, int EPT, int ND>
+template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __device__ __forceinline__ void j_gamma_kernel_body(
     const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
     const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart, int bx, int by) {
@@ -130,8 +154,8 @@ __device__ __forceinline__ void j_gamma_kernel_body(
             Vec<TB, VEC> b0[EPT], b1[EPT];
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
-                b0[k] = load_vec<TB, VEC>(B0 + off[k]);
-                b1[k] = load_vec<TB, VEC>(B1 + off[k]);
+                b0[k] = stream_vec<TB, VEC, NT>(B0 + off[k]);
+                b1[k] = stream_vec<TB, VEC, NT>(B1 + off[k]);
             }
             double a0[ND], a1[ND];
 #pragma unroll
@@ -165,7 +189,7 @@ __device__ __forceinline__ void j_gamma_kernel_body(
             for (int dd = 0; dd < ND; ++dd) a0[dd] = 0.0;
 #pragma unroll
             for (int k = 0; k < EPT; ++k) {
-                Vec<TB, VEC> b0 = load_vec<TB, VEC>(B0 + off[k]);
+                Vec<TB, VEC> b0 = stream_vec<TB, VEC, NT>(B0 + off[k]);
 #pragma unroll
                 for (int v = 0; v < VEC; ++v)
 #pragma unroll
@@ -190,11 +214,11 @@ __device__ __forceinline__ void j_gamma_kernel_body(
         __syncthreads();
     }
 }
-template <typename TB, int VEC, int EPT, int ND>
+template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
     const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
     const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart) {
-    j_gamma_kernel_body<TB, VEC, EPT, ND>(B, N, Q, D, ldd, rowoff, nvec, ppw, gpart, blockIdx.x, blockIdx.y);
+    j_gamma_kernel_body<TB, VEC, EPT, ND, NT>(B, N, Q, D, ldd, rowoff, nvec, ppw, gpart, blockIdx.x, blockIdx.y);
 }
 
 
@@ -213,7 +237,7 @@ __global__ void j_gamma_reduce_kernel(const double* __restrict__ gpart, int ngro
 // J pass 2: partial J per (P chunk, density, element).
 // grid = (n_element_groups, n_p_chunks), block = 256.
 // ---------------------------------------------------------------------------
-template <typename TB, int VEC, int EPT, int ND>
+template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __device__ __forceinline__ void j_accum_kernel_body(
     const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
     const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart, int bx, int by) {
@@ -248,8 +272,8 @@ __device__ __forceinline__ void j_accum_kernel_body(
         Vec<TB, VEC> b0[EPT], b1[EPT];
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            b0[k] = load_vec<TB, VEC>(B0 + off[k]);
-            b1[k] = load_vec<TB, VEC>(B1 + off[k]);
+            b0[k] = stream_vec<TB, VEC, NT>(B0 + off[k]);
+            b1[k] = stream_vec<TB, VEC, NT>(B1 + off[k]);
         }
         double g0[ND], g1[ND];
 #pragma unroll
@@ -271,7 +295,7 @@ __device__ __forceinline__ void j_accum_kernel_body(
         const TB* B0 = B + (int64_t)P * NN;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
-            Vec<TB, VEC> b0 = load_vec<TB, VEC>(B0 + off[k]);
+            Vec<TB, VEC> b0 = stream_vec<TB, VEC, NT>(B0 + off[k]);
 #pragma unroll
             for (int dd = 0; dd < ND; ++dd) {
                 const double g = gamma[(int64_t)dd * Q + P];
@@ -291,11 +315,11 @@ __device__ __forceinline__ void j_accum_kernel_body(
         }
     }
 }
-template <typename TB, int VEC, int EPT, int ND>
+template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __global__ __launch_bounds__(kJThreads) void j_accum_kernel(
     const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
     const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart) {
-    j_accum_kernel_body<TB, VEC, EPT, ND>(B, N, Q, gamma, rowoff, nvec, ppw, jpart, blockIdx.x, blockIdx.y);
+    j_accum_kernel_body<TB, VEC, EPT, ND, NT>(B, N, Q, gamma, rowoff, nvec, ppw, jpart, blockIdx.x, blockIdx.y);
 }
 
 
@@ -377,8 +401,8 @@ __device__ __forceinline__ void kx_kernel_body(const T* __restrict__ A, int64_t 
                                                        int ncolt, int b, int nblocks) {
     typedef typename Acc<T>::type acc_t;
     constexpr int VB = 16 / sizeof(T);       // B tile loads are always 16 B (OPAD % 16 == 0)
-    __shared__ T As[kTile][kBK + 1];
-    __shared__ T Bs[kBK][kTile];
+    __shared__ T As[2][kTile][kBK + 1];  // two stages: MFMAs read one, the next k-step fills the other
+    __shared__ T Bs[2][kBK][kTile];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -422,20 +446,20 @@ __device__ __forceinline__ void kx_kernel_body(const T* __restrict__ A, int64_t 
             }
         }
     };
-    auto store_tiles = [&]() {
+    auto store_tiles = [&](int stg) {
 #pragma unroll
         for (int s = 0; s < A_LD; ++s) {
             const int idx = tid + kGThreads * s;
             const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
 #pragma unroll
-            for (int v = 0; v < VA; ++v) As[row][kc + v] = ra[s].v[v];
+            for (int v = 0; v < VA; ++v) As[stg][row][kc + v] = ra[s].v[v];
         }
 #pragma unroll
         for (int s = 0; s < B_LD; ++s) {
             const int idx = tid + kGThreads * s;
             const int kr = idx / (kTile / VB), cc = (idx % (kTile / VB)) * VB;
 #pragma unroll
-            for (int v = 0; v < VB; ++v) Bs[kr][cc + v] = rb[s].v[v];
+            for (int v = 0; v < VB; ++v) Bs[stg][kr][cc + v] = rb[s].v[v];
         }
     };
 
@@ -447,17 +471,18 @@ __device__ __forceinline__ void kx_kernel_body(const T* __restrict__ A, int64_t 
 
     const int nk = (K + kBK - 1) / kBK;
     load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+    if (nk > 1) load_tiles(kBK);
     for (int kt = 0; kt < nk; ++kt) {
-        store_tiles();
-        __syncthreads();
-        if (kt + 1 < nk) load_tiles((kt + 1) * kBK);
+        const int cur = kt & 1;
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 4) {
             T a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+            for (int i = 0; i < 4; ++i) a[i] = As[cur][wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 64 + j * 16 + (lane & 15)];
+            for (int j = 0; j < 4; ++j) b[j] = Bs[cur][kk + (lane >> 4)][wc * 64 + j * 16 + (lane & 15)];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (j < nbv) {
@@ -466,6 +491,8 @@ __device__ __forceinline__ void kx_kernel_body(const T* __restrict__ A, int64_t 
                 }
             }
         }
+        if (kt + 1 < nk) store_tiles(cur ^ 1);  // the other stage was last read before the previous barrier
+        if (kt + 2 < nk) load_tiles((kt + 2) * kBK);
         __syncthreads();
     }
 #pragma unroll
@@ -502,8 +529,8 @@ __device__ __forceinline__ void ks_kernel_body(const T* __restrict__ X, int N, i
                                                        int accumulate, int b, int nblocks) {
     typedef typename Acc<T>::type acc_t;
     constexpr int VA = 16 / sizeof(T);
-    __shared__ T As[kTile][kBK + 1];
-    __shared__ T Bs[kTile][kBK + 1];
+    __shared__ T As[2][kTile][kBK + 1];  // two LDS stages
+    __shared__ T Bs[2][kTile][kBK + 1];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -541,15 +568,15 @@ __device__ __forceinline__ void ks_kernel_body(const T* __restrict__ X, int N, i
             }
         }
     };
-    auto store_tiles = [&]() {
+    auto store_tiles = [&](int stg) {
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
             const int idx = tid + kGThreads * q;
             const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
 #pragma unroll
             for (int v = 0; v < VA; ++v) {
-                As[row][kc + v] = ra[q].v[v];
-                Bs[row][kc + v] = rb[q].v[v];
+                As[stg][row][kc + v] = ra[q].v[v];
+                Bs[stg][row][kc + v] = rb[q].v[v];
             }
         }
     };
@@ -566,26 +593,29 @@ __device__ __forceinline__ void ks_kernel_body(const T* __restrict__ X, int N, i
 
     const int nkp = width / kBK;  // k steps per P (width % 16 == 0)
     const int nsteps = (Pb - Pa) * nkp;
-    if (nsteps > 0) load_tiles(Pa, 0);
+    auto load_step = [&](int st) { load_tiles(Pa + st / nkp, (st % nkp) * kBK); };
+    if (nsteps > 0) {
+        load_step(0);
+        store_tiles(0);
+    }
+    __syncthreads();
+    if (nsteps > 1) load_step(1);
     for (int st = 0; st < nsteps; ++st) {
-        store_tiles();
-        __syncthreads();
-        if (st + 1 < nsteps) {
-            const int sn = st + 1;
-            load_tiles(Pa + sn / nkp, (sn % nkp) * kBK);
-        }
+        const int cur = st & 1;
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 4) {
             T a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+            for (int i = 0; i < 4; ++i) a[i] = As[cur][wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[wc * 64 + j * 16 + (lane & 15)][kk + (lane >> 4)];
+            for (int j = 0; j < 4; ++j) b[j] = Bs[cur][wc * 64 + j * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = mfma4(a[i], b[j], acc[i][j]);
         }
+        if (st + 1 < nsteps) store_tiles(cur ^ 1);
+        if (st + 2 < nsteps) load_step(st + 2);
         __syncthreads();
         if constexpr (sizeof(T) == 4) {
             if ((st + 1) % nkp == 0) {  // end of one P: fold FP32 partials into FP64
@@ -683,10 +713,11 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
     const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt, int b, int nblocks) {
     constexpr int BN = 32 * NBW;
-    __shared__ __bf16 Ah[kTile][kPitch16];
-    __shared__ __bf16 Al[kTile][kPitch16];
-    __shared__ __bf16 Bh[BN][kPitch16];
-    __shared__ __bf16 Bl[BN][kPitch16];
+    // two LDS stages: the MFMAs read one while the next k-step is written to the other
+    __shared__ __bf16 Ah[2][kTile][kPitch16];
+    __shared__ __bf16 Al[2][kTile][kPitch16];
+    __shared__ __bf16 Bh[2][BN][kPitch16];
+    __shared__ __bf16 Bl[2][BN][kPitch16];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -732,20 +763,21 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
             }
         }
     };
-    auto store_tiles = [&]() {
+    auto store_tiles = [&](int stg) {
 #pragma unroll
         for (int s = 0; s < A_LD; ++s) {
             const int idx = tid + kGThreads * s;
             const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
 #pragma unroll
-            for (int v = 0; v < VA; ++v) split_bf16(ra[s].v[v], Ah[row][kc + v], Al[row][kc + v]);
+            for (int v = 0; v < VA; ++v)
+                split_bf16(ra[s].v[v], Ah[stg][row][kc + v], Al[stg][row][kc + v]);
         }
 #pragma unroll
         for (int s = 0; s < B_LD; ++s) {
             const int idx = tid + kGThreads * s;
             const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
-            *reinterpret_cast<uint4*>(&Bh[row][kc]) = rbh[s];
-            *reinterpret_cast<uint4*>(&Bl[row][kc]) = rbl[s];
+            *reinterpret_cast<uint4*>(&Bh[stg][row][kc]) = rbh[s];
+            *reinterpret_cast<uint4*>(&Bl[stg][row][kc]) = rbl[s];
         }
     };
 
@@ -757,15 +789,16 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 
     const int nk = KP / kBK16;
     load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+    if (nk > 1) load_tiles(kBK16);
     for (int kt = 0; kt < nk; ++kt) {
-        store_tiles();
-        __syncthreads();
-        if (kt + 1 < nk) load_tiles((kt + 1) * kBK16);
+        const int cur = kt & 1;
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-            al[i] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
         }
         // Every block is computed, also those past OPAD (their B rows are zero),
         // so the MFMA sequence has no branches and the accumulators stay in
@@ -774,8 +807,8 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 #pragma unroll
         for (int j = 0; j < NBW; ++j) {
             const int br = wc * 16 * NBW + j * 16 + (lane & 15);
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bh[br][8 * (lane >> 4)]);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bl[br][8 * (lane >> 4)]);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bh[cur][br][8 * (lane >> 4)]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bl[cur][br][8 * (lane >> 4)]);
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
 #pragma unroll
@@ -783,6 +816,9 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
         }
+        // the other stage was last read before the previous barrier
+        if (kt + 1 < nk) store_tiles(cur ^ 1);
+        if (kt + 2 < nk) load_tiles((kt + 2) * kBK16);
         __syncthreads();
     }
 #pragma unroll
@@ -817,10 +853,10 @@ __global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
 __device__ __forceinline__ void ks_bf16x3_kernel_body(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
     int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate, int b, int nblocks) {
-    __shared__ __bf16 Ah[kTile][kPitch16];
-    __shared__ __bf16 Al[kTile][kPitch16];
-    __shared__ __bf16 Bh[kTile][kPitch16];
-    __shared__ __bf16 Bl[kTile][kPitch16];
+    __shared__ __bf16 Ah[2][kTile][kPitch16];
+    __shared__ __bf16 Al[2][kTile][kPitch16];
+    __shared__ __bf16 Bh[2][kTile][kPitch16];
+    __shared__ __bf16 Bl[2][kTile][kPitch16];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -862,15 +898,15 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
             }
         }
     };
-    auto store_tiles = [&]() {
+    auto store_tiles = [&](int stg) {
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
             const int idx = tid + kGThreads * q;
             const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
-            *reinterpret_cast<uint4*>(&Ah[row][kc]) = rah[q];
-            *reinterpret_cast<uint4*>(&Al[row][kc]) = ral[q];
-            *reinterpret_cast<uint4*>(&Bh[row][kc]) = rbh[q];
-            *reinterpret_cast<uint4*>(&Bl[row][kc]) = rbl[q];
+            *reinterpret_cast<uint4*>(&Ah[stg][row][kc]) = rah[q];
+            *reinterpret_cast<uint4*>(&Al[stg][row][kc]) = ral[q];
+            *reinterpret_cast<uint4*>(&Bh[stg][row][kc]) = rbh[q];
+            *reinterpret_cast<uint4*>(&Bl[stg][row][kc]) = rbl[q];
         }
     };
 
@@ -885,27 +921,32 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         }
 
     const int nkp = width / kBK16;
-    if (Pb > Pa) load_tiles(Pa, 0);
+    // steps are numbered across P; step st+1's tiles are written to the other
+    // LDS stage while the MFMAs of step st run
+    const int nsteps = (Pb - Pa) * nkp;
+    auto load_step = [&](int st) { load_tiles(Pa + st / nkp, (st % nkp) * kBK16); };
+    if (nsteps > 0) {
+        load_step(0);
+        store_tiles(0);
+    }
+    __syncthreads();
+    if (nsteps > 1) load_step(1);
+    int st = 0;
     for (int P = Pa; P < Pb; ++P) {
-    for (int kt = 0; kt < nkp; ++kt) {
-        store_tiles();
-        __syncthreads();
-        if (kt + 1 < nkp)
-            load_tiles(P, (kt + 1) * kBK16);
-        else if (P + 1 < Pb)
-            load_tiles(P + 1, 0);
+    for (int kt = 0; kt < nkp; ++kt, ++st) {
+        const int cur = st & 1;
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-            al[i] = *reinterpret_cast<const bf16x8*>(&Al[wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
         }
         bf16x8 bh[4], bl[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int br = wc * 64 + j * 16 + (lane & 15);
-            bh[j] = *reinterpret_cast<const bf16x8*>(&Bh[br][8 * (lane >> 4)]);
-            bl[j] = *reinterpret_cast<const bf16x8*>(&Bl[br][8 * (lane >> 4)]);
+            bh[j] = *reinterpret_cast<const bf16x8*>(&Bh[cur][br][8 * (lane >> 4)]);
+            bl[j] = *reinterpret_cast<const bf16x8*>(&Bl[cur][br][8 * (lane >> 4)]);
         }
         // lo*hi, hi*lo, then hi*hi: each sweep touches all 16 accumulators once
 #pragma unroll
@@ -920,6 +961,8 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+        if (st + 1 < nsteps) store_tiles(cur ^ 1);
+        if (st + 2 < nsteps) load_step(st + 2);
         __syncthreads();
     }
     // end of one P: fold the FP32 partials into FP64

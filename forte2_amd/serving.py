@@ -34,6 +34,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import autotune
 from . import df as _df
 from .jkbuilder import FockBuilder
 from .scf import RHF, UHF, DeviceDIIS
@@ -186,6 +187,7 @@ class SCFServer:
         system.device = self.device
         B = _df.device_B_from_host(M, T, self.device)
         fb = FockBuilder(system, B=B, device=self.device)
+        autotune.apply(fb, "fp64")  # launch shape of the job's size class
         system.fock_builder = fb
         kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter)
         scf = RHF(req.charge, **kw) if req.ms is None else UHF(req.charge, ms=req.ms, **kw)

@@ -110,3 +110,20 @@ def test_generalized_fock_matches_oracle(dev):
     H = s.ints_hcore()
     assert np.abs(fb.build_core_fock(C[:, :2], H) - ojk.build_core_fock(B, C[:, :2], H)).max() < 1e-10
     assert np.abs(fb.build_active_fock(C, g1) - ojk.build_active_fock(B, C, g1)).max() < 1e-10
+
+
+def test_metric_inverse_sqrt_branch_gives_same_jk(dev):
+    # df_ortho_rtol selects X = M^-1/2 (jkbuilder.py:137-139); J and K do not
+    # depend on the choice of metric factor when nothing is discarded
+    from forte2_amd.df import build_B_density_fitting
+    from forte2_amd.jkbuilder import FockBuilder
+
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    Bc = build_B_density_fitting(s, dev)
+    s.df_ortho_rtol = 1e-14
+    Bs = build_B_density_fitting(s, dev)
+    C = torch.linalg.qr(torch.randn(s.nbf, 5, dtype=torch.float64, generator=torch.Generator().manual_seed(1)))[0].to(dev)
+    Jc, Kc = FockBuilder(B=Bc, device=dev).build_JK([C])
+    Js, Ks = FockBuilder(B=Bs, device=dev).build_JK([C])
+    assert float((Jc[0] - Js[0]).abs().max()) < 1e-10
+    assert float((Kc[0] - Ks[0]).abs().max()) < 1e-10
