@@ -28,8 +28,6 @@ import time
 
 import torch
 
-from .jkbuilder import FockBuilder
-
 
 # ---------------------------------------------------------------- helpers
 def canonical_orth(S, rtol=1e-8):
@@ -122,6 +120,7 @@ class _SCF:
         self.log = log
         self.converged = False
         self.executed = False
+        self.V_sap = None  # SAP guess potential, computed from the system when not given
 
     def __call__(self, system):
         self.system = system
@@ -136,8 +135,9 @@ class _SCF:
     def _setup(self):
         sysm = self.system
         fb = sysm.fock_builder
-        if not isinstance(fb, FockBuilder):
-            raise TypeError("system.fock_builder must be a forte2_amd FockBuilder")
+        # duck-typed, as forte2's own SCF (scf_base.py:143): build_JK/J/K and a device
+        if not all(hasattr(fb, a) for a in ("build_JK", "build_J", "build_K", "device")):
+            raise TypeError("system.fock_builder lacks build_JK / build_J / build_K / device")
         self.fb = fb
         dev = fb.device
         self.dev = dev
@@ -156,50 +156,74 @@ class _SCF:
 
     def _guess(self):
         if self.guess_type == "minao":
-            V = torch.as_tensor(self.system.sap_potential(), dtype=torch.float64).to(self.dev)
+            Vh = self.V_sap if self.V_sap is not None else self.system.sap_potential()
+            V = torch.as_tensor(Vh, dtype=torch.float64).to(self.dev)
             return self._eigh(self.H + V)[1]
         if self.guess_type == "hcore":
             return self._eigh(self.H)[1]
         raise RuntimeError(f"Unknown initial guess type: {self.guess_type}")
 
     # ------------------------------------------------------------------ precision schedule
-    def _fock_for(self, stage):
-        """Fock matrices for the current orbitals in the given stage."""
+    # The SCF loop is a generator: it yields the J/K builds it needs as
+    # ("JK", C_occ list, precision), ("J", D list, precision) or
+    # ("K", C list, precision) and receives their results.  run() executes
+    # every request at once; the serving layer batches the FP64 "JK"
+    # requests of many jobs into grouped launches.
+    def _fock_gen(self, stage):
+        """Yields the builds of the Fock matrices of `stage`; returns them."""
         Cocc = self._occ()
-        if stage in ("fp64", "lp"):
-            prec = "fp64" if stage == "fp64" else self.precision
-            J, K = self.fb.build_JK(Cocc, precision=prec)
-            return self._assemble(J, K)
-        if stage == "ref":
-            J, K = self.fb.build_JK(Cocc, precision="fp64")
+        if stage in ("fp64", "lp", "ref"):
+            prec = self.precision if stage == "lp" else "fp64"
+            J, K = yield ("JK", Cocc, prec)
             F = self._assemble(J, K)
-            self._ref = ([c.clone() for c in Cocc], [d.clone() for d in self.D], [f.clone() for f in F])
+            if stage == "ref":
+                self._ref = ([c.clone() for c in Cocc], [d.clone() for d in self.D], [f.clone() for f in F])
             return F
         # incremental: F = F_ref + G_lp(D - D_ref)
         Cref, Dref, Fref = self._ref
         dD = [d - dr for d, dr in zip(self.D, Dref)]
-        Jd = self.fb.build_J([sum(dD)] if len(dD) > 1 else dD, precision=self.precision)[0]
+        Jd = (yield ("J", [sum(dD)] if len(dD) > 1 else dD, self.precision))[0]
         Kd = []
         for cn, co in zip(Cocc, Cref):
             split = _procrustes_split(cn, co, self.S)
             if split is None:
                 Kd.append(torch.zeros_like(Jd))
             else:
-                Kp, Km = self.fb.build_K(list(split), precision=self.precision)
+                Kp, Km = yield ("K", list(split), self.precision)
                 Kd.append(0.5 * (Kp - Km))
         return self._assemble_delta(Fref, Jd, Kd)
 
+    def _execute(self, req):
+        kind, args, prec = req
+        if kind == "JK":
+            return self.fb.build_JK(args, precision=prec)
+        if kind == "J":
+            return self.fb.build_J(args, precision=prec)
+        return self.fb.build_K(args, precision=prec)
+
     def run(self):
+        gen = self.iterate()
+        try:
+            req = next(gen)
+            while True:
+                req = gen.send(self._execute(req))
+        except StopIteration:
+            pass
+        return self
+
+    def iterate(self):
+        """The SCF loop of scf_base.py:121-251 as a generator of J/K build requests."""
         t0 = time.monotonic()
         self._setup()
         diis = DeviceDIIS(self.diis_start, self.diis_nvec, self.diis_min, self.do_diis)
         self.iter = 0
         self.history = []
+        self._last_dE = float("inf")
         stage = "fp64" if self.precision == "fp64" else "lp"
         if self.C is None:
             self.C = self._guess_orbitals()
         self.D = self._density()
-        F = self._fock_for(stage)
+        F = yield from self._fock_gen(stage)
         self.E = self.Vnn + self._energy(F)
         Eold, Dold = self.E, self.D
         self.iter += 1
@@ -216,13 +240,13 @@ class _SCF:
                 next_stage = "ref" if self.incremental else "fp64"
             elif stage == "inc" and abs(self._last_dE) < self.switch_fp64:
                 next_stage = "fp64"
-            if stage == "lp" and next_stage == "fp64" or stage == "inc" and next_stage == "fp64":
-                diis.reset()
-            F = self._fock_for(next_stage)
+            if stage == "lp" and next_stage == "fp64":
+                diis.reset()  # low-precision Fock matrices leave the DIIS subspace
+            F = yield from self._fock_gen(next_stage)
             self.E = self.Vnn + self._energy(F)
             if next_stage == "ref":
                 # error of the low-precision build against FP64 at the same density
-                Flp = self._fock_for("lp")
+                Flp = yield from self._fock_gen("lp")
                 self.lp_error_vs_fp64 = abs((self.Vnn + self._energy(Flp)) - self.E)
                 diis.reset()
                 next_stage = "inc"
@@ -236,7 +260,7 @@ class _SCF:
             if stage == "fp64" and abs(dE) < self.e_tol and dD < self.d_tol:
                 self.eps, self.C = self._diagonalize(Fx)
                 self.D = self._density()
-                F = self._fock_for("fp64")
+                F = yield from self._fock_gen("fp64")
                 self.E = self.Vnn + self._energy(F)
                 self.converged = True
                 break
@@ -251,7 +275,6 @@ class _SCF:
         self.executed = True
         return self
 
-    _last_dE = float("inf")
 
 
 class RHF(_SCF):

@@ -37,7 +37,7 @@ from . import _native as nat
 from . import autotune
 from . import df as _df
 from .jkbuilder import FockBuilder
-from .scf import RHF, UHF, DeviceDIIS
+from .scf import RHF, UHF
 from .system import System
 
 MAX_GROUP = 12
@@ -83,64 +83,27 @@ def job_footprint(nbf, naux, nocc_total):
 
 
 class _Job:
-    """One SCF run as a state machine, driven from outside one J/K build at a time."""
+    """One SCF run driven one J/K build at a time.
+
+    The SCF loop itself is ``RHF/UHF.iterate()``: a generator that yields its
+    build requests ("JK" / "J" / "K", arguments, precision) and receives
+    their results, so the served job runs exactly the loop of ``run()``.
+    """
 
     def __init__(self, req, system, scf, fb):
         self.req, self.system, self.scf, self.fb = req, system, scf, fb
-        self.phase = "init"
-        self.Fx = None
+        self.error = None
+        self.gen = scf.iterate()
+        self.pending = next(self.gen)  # set-up, guess and the first build request
 
-    def occ(self):
-        return self.scf._occ()
-
-    def start(self, V_sap):
-        s = self.scf
-        s._setup()
-        s.C = s._guess_orbitals() if V_sap is None else self._sap_guess(V_sap)
-        s.D = s._density()
-        s.iter = 0
-        self.diis = DeviceDIIS(s.diis_start, s.diis_nvec, s.diis_min, s.do_diis)
-
-    def _sap_guess(self, V_sap):
-        s = self.scf
-        C = s._eigh(s.H + torch.as_tensor(V_sap, dtype=torch.float64).to(s.dev))[1]
-        return [C] if isinstance(s, RHF) else [C, C.clone()]
-
-    def _prepare_next(self, F):
-        s = self.scf
-        grad = s._ao_grad(F)
-        self.Fx = s._diis_update(self.diis, F, grad)
-        s.eps, s.C = s._diagonalize(self.Fx)
-        s.D = s._density()
-
-    def advance(self, J, K):
-        """Consume this job's J/K; returns True when the job is finished."""
-        s = self.scf
-        F = s._assemble(J, K)
-        s.E = s.Vnn + s._energy(F)
-        if self.phase == "init":
-            self.Eold, self.Dold = s.E, s.D
-            s.iter = 1
-            self.phase = "loop"
-            self._prepare_next(F)
+    def send(self, result):
+        """Deliver the result of the pending request; True when the job has finished."""
+        try:
+            self.pending = self.gen.send(result)
             return False
-        if self.phase == "final":
-            s.converged = True
-            s.F = F
+        except StopIteration:
+            self.pending = None
             return True
-        dE = s.E - self.Eold
-        dD = sum(float(torch.linalg.norm(d - do)) for d, do in zip(s.D, self.Dold))
-        if abs(dE) < s.e_tol and dD < s.d_tol:
-            s.eps, s.C = s._diagonalize(self.Fx)
-            s.D = s._density()
-            self.phase = "final"
-            return False
-        self.Eold, self.Dold = s.E, s.D
-        s.iter += 1
-        if s.iter > s.maxiter:
-            raise RuntimeError(f"{s.method} did not converge in {s.maxiter} iterations.")
-        self._prepare_next(F)
-        return False
 
 
 class SCFServer:
@@ -154,10 +117,11 @@ class SCFServer:
     prep_threads : host threads computing the one-time integrals of admitted jobs
     grouped : batch the J/K builds of a tick into grouped launches
     e_tol, d_tol, maxiter : SCF convergence (forte2 semantics)
+    precision : "fp64", or a precision form ("bf16x3", "fp32") for every job
     """
 
     def __init__(self, device=None, hbm_budget=None, max_active=12, prep_threads=4, grouped=True,
-                 e_tol=1e-8, d_tol=1e-6, maxiter=100, log=None):
+                 e_tol=1e-8, d_tol=1e-6, maxiter=100, precision="fp64", log=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -167,6 +131,7 @@ class SCFServer:
         self.prep_threads = int(prep_threads)
         self.grouped = grouped
         self.e_tol, self.d_tol, self.maxiter = e_tol, d_tol, maxiter
+        self.precision = precision
         self.log = log
         self.results = {}
         self.ticks = 0
@@ -189,12 +154,11 @@ class SCFServer:
         fb = FockBuilder(system, B=B, device=self.device)
         autotune.apply(fb, "fp64")  # launch shape of the job's size class
         system.fock_builder = fb
-        kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter)
+        kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter, precision=self.precision)
         scf = RHF(req.charge, **kw) if req.ms is None else UHF(req.charge, ms=req.ms, **kw)
         scf(system)
-        job = _Job(req, system, scf, fb)
-        job.start(V_sap)
-        return job
+        scf.V_sap = V_sap
+        return _Job(req, system, scf, fb)
 
     # ------------------------------------------------------------------ grouped J/K
     def _build_group(self, jobs):
@@ -203,7 +167,7 @@ class SCFServer:
         ctxs, nds, Cs, nocc, ldc, Js, Ks = [], [], [], [], [], [], []
         outs = []
         for jb in jobs:
-            occ = [c.contiguous() for c in jb.occ()]
+            occ = [c.contiguous() for c in jb.pending[1]]
             N = jb.fb.nbf
             J = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ]
             K = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ]
@@ -236,26 +200,36 @@ class SCFServer:
         return outs
 
     def _tick(self, active):
-        """One J/K build for every active job, then one SCF step each."""
-        done = []
-        by_nd = {}
+        """Serve the pending build of every active job, then let each take its next step.
+
+        FP64 full builds ("JK") of jobs with an even nbf are batched into
+        grouped launches (jobs with 1 and 2 densities separately); any other
+        request (precision-form builds, odd nbf) runs on its own.
+        """
+        done, results, solo = [], {}, []
+        groups = {}
         for jb in active:
-            by_nd.setdefault(len(jb.occ()), []).append(jb)
-        for nd, jobs in by_nd.items():
+            kind, args, prec = jb.pending
+            if self.grouped and kind == "JK" and prec == "fp64" and jb.fb.nbf % 2 == 0:
+                groups.setdefault(len(args), []).append(jb)
+            else:
+                solo.append(jb)
+        for jobs in groups.values():
             for g0 in range(0, len(jobs), MAX_GROUP):
                 grp = jobs[g0:g0 + MAX_GROUP]
-                if self.grouped:
-                    outs = self._build_group(grp)
-                else:
-                    outs = [jb.fb.build_JK(jb.occ()) for jb in grp]
+                for jb, out in zip(grp, self._build_group(grp)):
+                    results[id(jb)] = out
                 self.group_sizes.append(len(grp))
-                for jb, (J, K) in zip(grp, outs):
-                    try:
-                        if jb.advance(J, K):
-                            done.append(jb)
-                    except RuntimeError as e:
-                        jb.error = str(e)
-                        done.append(jb)
+        for jb in solo:
+            results[id(jb)] = jb.scf._execute(jb.pending)
+            self.group_sizes.append(1)
+        for jb in active:
+            try:
+                if jb.send(results[id(jb)]):
+                    done.append(jb)
+            except RuntimeError as e:  # non-convergence: this job fails, the others go on
+                jb.error = str(e)
+                done.append(jb)
         self.ticks += 1
         return done
 

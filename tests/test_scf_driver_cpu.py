@@ -1,0 +1,103 @@
+"""The device SCF driver (forte2_amd.scf) run on CPU tensors with a numpy-backed
+builder: checks the loop, the precision schedule and the build-request
+protocol without a GPU.  The builder's "low precision" rounds B to FP32."""
+
+import numpy as np
+import pytest
+import torch
+
+from forte2_amd.scf import RHF, UHF
+from forte2_amd.system import System
+from oracle import jk as ojk
+from oracle import scf as oscf
+
+WATER = """O 0.000000000000 0.000000000000 -0.061664597388
+H 0.000000000000 -0.711620616369 0.489330954643
+H 0.000000000000 0.711620616369 0.489330954643"""
+
+
+class HostBuilder:
+    """build_JK / build_J / build_K over the oracle, on CPU torch tensors."""
+
+    def __init__(self, B):
+        self.B = B
+        self.B32 = B.astype(np.float32).astype(np.float64)
+        self.device = torch.device("cpu")
+        self.calls = []
+
+    def _B(self, precision):
+        return self.B if precision == "fp64" else self.B32
+
+    def build_JK(self, C, precision="fp64"):
+        self.calls.append(("JK", precision))
+        J, K = ojk.build_JK(self._B(precision), [c.numpy() for c in C])
+        return [torch.from_numpy(j) for j in J], [torch.from_numpy(k) for k in K]
+
+    def build_J(self, D, precision="fp64"):
+        self.calls.append(("J", precision))
+        return [torch.from_numpy(j) for j in ojk.build_J(self._B(precision), [d.numpy() for d in D])]
+
+    def build_K(self, C, precision="fp64"):
+        self.calls.append(("K", precision))
+        return [torch.from_numpy(k) for k in ojk.build_K_nPm(self._B(precision), [c.numpy() for c in C])]
+
+
+@pytest.fixture(scope="module")
+def water():
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    return s, B
+
+
+def test_plain_form_matches_oracle(water):
+    s, B = water
+    s.fock_builder = HostBuilder(B)
+    r = RHF(charge=0, e_tol=1e-10)(s).run()
+    ref = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                       X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+    assert abs(r.E - ref.E) < 1e-10
+    assert r.iter == ref.iter
+
+
+def test_precision_schedule_and_requests(water):
+    s, B = water
+    hb = HostBuilder(B)
+    s.fock_builder = hb
+    r = RHF(charge=0, e_tol=1e-10, precision="fp32")(s).run()
+    e64 = RHF(charge=0, e_tol=1e-10)(s).run().E
+    stages = "".join(h[1][0] for h in r.history)
+    assert stages.startswith("l") and "i" in stages and stages.endswith("f")
+    assert abs(r.E - e64) < 1e-9
+    assert r.lp_error_vs_fp64 is not None and r.lp_error_vs_fp64 > 0
+    kinds = {c for c in hb.calls}
+    assert ("JK", "fp32") in kinds and ("J", "fp32") in kinds and ("K", "fp32") in kinds
+    assert ("JK", "fp64") in kinds
+
+
+def test_generator_protocol_yields_build_requests(water):
+    s, B = water
+    hb = HostBuilder(B)
+    s.fock_builder = hb
+    scf = RHF(charge=0, e_tol=1e-8)(s)
+    gen = scf.iterate()
+    req = next(gen)
+    assert req[0] == "JK" and req[2] == "fp64" and req[1][0].shape == (s.nbf, 5)
+    n = 0
+    try:
+        while True:
+            req = gen.send(scf._execute(req))
+            n += 1
+    except StopIteration:
+        pass
+    assert scf.converged and n == scf.iter + 1
+
+
+def test_uhf_cation_on_host_builder():
+    s = System("H 0 0 0\nH 0 0 0.74\nH 0 0.9 0.37", "cc-pvdz", "cc-pvtz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    s.fock_builder = HostBuilder(B)
+    r = UHF(charge=0, ms=0.5, e_tol=1e-10)(s).run()
+    ref = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum, 0.5,
+                       X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+    assert abs(r.E - ref.E) < 1e-10
+    assert abs(r.S2 - ref.S2) < 1e-8

@@ -26,16 +26,11 @@ def test_grouped_build_is_bitwise_single_build(dev):
         fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=i, device=dev), device=dev))
         Cs.append([orthonormal_C(N, o, seed=10 * i + k, device=dev) for k, o in enumerate(noccs)])
     singles = [fb.build_JK(C) for fb, C in zip(fbs, Cs)]
-    from forte2_amd.serving import _Job  # noqa: F401  (module import check)
-
     srv = SCFServer(device=dev)
 
     class _Fake:
         def __init__(self, fb, C):
-            self.fb, self._C = fb, C
-
-        def occ(self):
-            return self._C
+            self.fb, self.pending = fb, ("JK", C, "fp64")
 
     # mixed group (one density and two densities are grouped separately by the server)
     one = [_Fake(fb, C) for fb, C in zip(fbs, Cs) if len(C) == 1]
@@ -103,3 +98,13 @@ def test_stream_with_arrivals_and_tight_memory(dev):
     assert summ["completed"] == len(reqs)
     assert all(r.latency >= 0 for r in res.values())
     assert summ["latency_p99_s"] >= summ["latency_p50_s"]
+
+
+def test_precision_form_in_the_mix(dev):
+    reqs = _requests()[:3]
+    srv = SCFServer(device=dev, e_tol=1e-10, precision="bf16x3")
+    mix = srv.run(reqs)
+    ref = SCFServer(device=dev, e_tol=1e-10).run([SCFRequest(r.id, r.xyz, arrival=0.0) for r in reqs])
+    for r in reqs:
+        assert mix[r.id].converged
+        assert abs(mix[r.id].E - ref[r.id].E) <= 1e-8
