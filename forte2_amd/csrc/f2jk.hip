@@ -57,7 +57,12 @@ struct f2jk_ctx {
     int* rowoff[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // by VEC (1, 2, 4)
     int nvec[5] = {0, 0, 0, 0, 0};
     size_t xchunk_bytes = kDefaultXChunkBytes;
-    bool j_nontemporal = false;
+    bool j_nontemporal = true;
+    int bf16_nbw_max = 10;          // widest BF16x3 half-transform tile: 32 * 10 columns
+    // J (HBM-bound) runs on a side stream concurrently with K (MFMA-bound)
+    bool overlap_jk = true;
+    hipStream_t side = nullptr;     // default priority, created with the context
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int target_blocks = kTargetBlocks;
     // timing
     hipEvent_t ev[kMaxEvents];      // (start, end) pairs of non-nested phases
@@ -198,7 +203,7 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         p.nlower = p.ntile * (p.ntile + 1) / 2;
         p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(ctx->target_blocks / 2, p.nlower)));
         if (precision == F2JK_PREC_BF16X3) {
-            const int need = (int)cdiv(std::min(p.OPAD, 320), 32);
+            const int need = (int)cdiv(std::min(p.OPAD, 32 * ctx->bf16_nbw_max), 32);
             p.nbw = need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : 10;
             p.ncolt = (int)cdiv(p.OPAD, 32 * p.nbw);
         } else {
@@ -505,13 +510,21 @@ int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t
             if ((rc = build_d(ctx, p, w, st))) return rc;
         TIM(F2JK_PH_PREP, false);
     }
+    // fork: the J passes go to the side stream once the prepared inputs exist
+    const bool overlap = p.want_j && p.want_k && ctx->overlap_jk;
+    hipStream_t sj = st;
+    if (overlap) {
+        F2JK_HIP(hipEventRecord(ctx->ev_fork, st));
+        F2JK_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        sj = ctx->side;
+    }
     if (p.want_j) {
         std::vector<const double*> dptr(ndens);
         int64_t lddu = ldd;
         for (int d = 0; d < ndens; ++d)
             dptr[d] = j_from_c ? (const double*)(w + p.off_d) + (size_t)d * p.N * p.N : Din[d];
         if (j_from_c) lddu = p.N;
-        if ((rc = run_j(ctx, p, dptr.data(), lddu, J, w, st))) return rc;
+        if ((rc = run_j(ctx, p, dptr.data(), lddu, J, w, sj))) return rc;
     }
     if (p.want_k) {
         if (p.prec == F2JK_PREC_BF16X3)
@@ -519,6 +532,11 @@ int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t
         else
             rc = (p.tk == 8) ? run_k_typed<double>(ctx, p, K, w, st) : run_k_typed<float>(ctx, p, K, w, st);
         if (rc) return rc;
+    }
+    // join: later work on the caller's stream sees J
+    if (overlap) {
+        F2JK_HIP(hipEventRecord(ctx->ev_join, sj));
+        F2JK_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
     }
     TIM(F2JK_PH_TOTAL, false);
     g_last_error.clear();
@@ -583,6 +601,13 @@ int f2jk_ctx_create(int device, f2jk_ctx** out) {
             return fail(F2JK_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
         }
     }
+    hipError_t e1 = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    hipError_t e2 = (e1 == hipSuccess) ? hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) : e1;
+    hipError_t e3 = (e2 == hipSuccess) ? hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) : e2;
+    if (e3 != hipSuccess) {
+        f2jk_ctx_destroy(ctx);
+        return fail(F2JK_EHIP, std::string("side stream: ") + hipGetErrorString(e3));
+    }
     *out = ctx;
     return F2JK_OK;
 }
@@ -592,6 +617,12 @@ int f2jk_ctx_destroy(f2jk_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     for (int i = 0; i < kMaxEvents; ++i) (void)hipEventDestroy(ctx->ev[i]);
     for (int i = 0; i < 2; ++i) (void)hipEventDestroy(ctx->ev_total[i]);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
     for (int v = 0; v < 5; ++v)
         if (ctx->rowoff[v]) (void)hipFree(ctx->rowoff[v]);
     delete ctx;
@@ -641,6 +672,12 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
         ctx->xchunk_bytes = (size_t)value;
     } else if (!strcmp(key, "j_nontemporal")) {
         ctx->j_nontemporal = value != 0;
+    } else if (!strcmp(key, "bf16_nbw_max")) {
+        if (value != 2 && value != 4 && value != 6 && value != 8 && value != 10)
+            return fail(F2JK_EINVAL, "bf16_nbw_max must be 2, 4, 6, 8 or 10");
+        ctx->bf16_nbw_max = (int)value;
+    } else if (!strcmp(key, "overlap_jk")) {
+        ctx->overlap_jk = value != 0;
     } else if (!strcmp(key, "target_blocks")) {
         if (value < 1 || value > (1 << 20)) return fail(F2JK_EINVAL, "target_blocks out of range");
         ctx->target_blocks = (int)value;
