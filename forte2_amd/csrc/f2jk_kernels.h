@@ -732,13 +732,10 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     constexpr int A_LD = (kTile * kBK16) / (kGThreads * VA);
     constexpr int B_LD = (BN * kBK16 / 8) / kGThreads;   // 16-byte loads per array
     static_assert((BN * kBK16 / 8) % kGThreads == 0, "B tile must split evenly");
-    // A (streamed from HBM) is prefetched two k-steps ahead in two register
-    // sets; the coefficient tile (L2-resident) one step ahead, just before the
-    // MFMA sweep that hides its latency.
-    Vec<float, VA> ra0[A_LD], ra1[A_LD];
+    Vec<float, VA> ra[A_LD];
     uint4 rbh[B_LD], rbl[B_LD];
 
-    auto load_a = [&](int k0, Vec<float, VA>* ra) {
+    auto load_tiles = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < A_LD; ++s) {
             const int idx = tid + kGThreads * s;
@@ -752,8 +749,6 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
                 for (int v = 0; v < VA; ++v) ra[s].v[v] = 0.f;
             }
         }
-    };
-    auto load_b = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < B_LD; ++s) {
             const int idx = tid + kGThreads * s;
@@ -768,7 +763,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
             }
         }
     };
-    auto store_tiles = [&](int stg, const Vec<float, VA>* ra) {
+    auto store_tiles = [&](int stg) {
 #pragma unroll
         for (int s = 0; s < A_LD; ++s) {
             const int idx = tid + kGThreads * s;
@@ -792,17 +787,23 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 #pragma unroll
         for (int j = 0; j < NBW; ++j) acc[i][j] = fx4{0, 0, 0, 0};
 
-    auto mfma_step = [&](int cur) {
+    const int nk = KP / kBK16;
+    load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+    if (nk > 1) load_tiles(kBK16);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
         bf16x8 ah[4], al[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
             al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
         }
-        // Every block is computed, also those past OPAD (their coefficient rows
-        // are zero), so the MFMA sequence has no branches; the three products
-        // of a block are issued in separate sweeps so that consecutive MFMAs
-        // never depend on each other.
+        // Every block is computed, also those past OPAD (their B rows are zero),
+        // so the MFMA sequence has no branches and the accumulators stay in
+        // place; the three products of a block are issued in separate sweeps
+        // so that consecutive MFMAs never depend on each other.
 #pragma unroll
         for (int j = 0; j < NBW; ++j) {
             const int br = wc * 16 * NBW + j * 16 + (lane & 15);
@@ -815,27 +816,10 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
         }
-    };
-    // one k-step: A of step kt+2 into `load_into`, the coefficients of kt+1,
-    // the MFMAs of kt, then the LDS fill of kt+1 from `store_from`
-    auto step = [&](int kt, int nk, Vec<float, VA>* load_into, const Vec<float, VA>* store_from) {
-        const int cur = kt & 1;
-        if (kt + 2 < nk) load_a((kt + 2) * kBK16, load_into);
-        if (kt + 1 < nk) load_b((kt + 1) * kBK16);
-        mfma_step(cur);
-        if (kt + 1 < nk) store_tiles(cur ^ 1, store_from);  // the other stage was last read before the previous barrier
+        // the other stage was last read before the previous barrier
+        if (kt + 1 < nk) store_tiles(cur ^ 1);
+        if (kt + 2 < nk) load_tiles((kt + 2) * kBK16);
         __syncthreads();
-    };
-
-    const int nk = KP / kBK16;
-    load_a(0, ra0);
-    load_b(0);
-    store_tiles(0, ra0);
-    __syncthreads();
-    if (nk > 1) load_a(kBK16, ra1);
-    for (int kt = 0; kt < nk; kt += 2) {
-        step(kt, nk, ra0, ra1);
-        if (kt + 1 < nk) step(kt + 1, nk, ra1, ra0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
