@@ -29,6 +29,10 @@ HIPCC_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # per-thread staging arrays stay in registers: with LDS to spare the
+    # compiler would otherwise move them into shared memory
+    "-mllvm",
+    "-disable-promote-alloca-to-lds",
 ]
 GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-march=x86-64-v3"]
 

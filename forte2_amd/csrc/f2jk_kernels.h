@@ -22,6 +22,7 @@
 // All reductions run in a fixed order (no atomics): the result of a build is
 // a function of its inputs and shapes only.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -674,9 +675,42 @@ __global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
+// 16-byte staging registers: a native vector (plain loads/stores, no struct copies)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBK16 = 32;              // k extent of one LDS stage = one MFMA k step
-constexpr int kPitch16 = kBK16 + 8;    // LDS row pitch (bf16): 80 B rows, conflict-free 16 B reads
+
+// LDS rows of the bf16 tiles are 32 elements (64 B) with the 16-byte chunks
+// swizzled: chunk c of row r sits at element (c ^ ((r >> 2) & 2)) * 8.  gfx950
+// serves a ds_read_b128 in passes of 32 lanes over 128 banks (512 B); a
+// fragment read (lanes 0..31 = rows 0..15, chunks 0..1) would hit rows r and
+// r + 8 in the same banks with plain 64 B (or 80 B) rows.  With the swizzle
+// both those reads and row-ordered 16-byte stores are conflict-free.
+__device__ __forceinline__ int swz16(int row, int chunk) { return (chunk ^ ((row >> 2) & 2)) << 3; }
+
+// f(integral_constant<int, I>) for I = B..E-1: loop bodies whose register
+// arrays are indexed by compile-time constants before any loop unrolling, so
+// that the arrays are split into registers and never placed in memory.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// A register prefetched in one iteration and consumed in the next is a phi of
+// two loads (prologue and loop body); the compiler folds such a phi into one
+// load at the loop head, right before the use, which removes the prefetch
+// distance.  Passing the prologue's values through an empty asm that claims
+// to modify them keeps the phi from being made of loads only (the cost is one
+// wait for the prologue loads).
+__device__ __forceinline__ void pin(u32x4& v) { asm volatile("" : "+v"(v)); }
+template <int VA>
+__device__ __forceinline__ void pin(Vec<float, VA>& v) {
+#pragma unroll
+    for (int i = 0; i < VA; ++i) asm volatile("" : "+v"(v.v[i]));
+}
 
 __device__ __forceinline__ fx4 mfma_bf16(bf16x8 a, bf16x8 b, fx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -714,10 +748,10 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt, int b, int nblocks) {
     constexpr int BN = 32 * NBW;
     // two LDS stages: the MFMAs read one while the next k-step is written to the other
-    __shared__ __bf16 Ah[2][kTile][kPitch16];
-    __shared__ __bf16 Al[2][kTile][kPitch16];
-    __shared__ __bf16 Bh[2][BN][kPitch16];
-    __shared__ __bf16 Bl[2][BN][kPitch16];
+    __shared__ __bf16 Ah[2][kTile][kBK16];
+    __shared__ __bf16 Al[2][kTile][kBK16];
+    __shared__ __bf16 Bh[2][BN][kBK16];
+    __shared__ __bf16 Bl[2][BN][kBK16];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -732,53 +766,50 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     constexpr int A_LD = (kTile * kBK16) / (kGThreads * VA);
     constexpr int B_LD = (BN * kBK16 / 8) / kGThreads;   // 16-byte loads per array
     static_assert((BN * kBK16 / 8) % kGThreads == 0, "B tile must split evenly");
+    static_assert(2 * B_LD <= NBW, "one B part per odd MFMA group");
     Vec<float, VA> ra[A_LD];
-    uint4 rbh[B_LD], rbl[B_LD];
+    u32x4 rbh[B_LD], rbl[B_LD];
 
-    auto load_tiles = [&](int k0) {
-#pragma unroll
-        for (int s = 0; s < A_LD; ++s) {
+    // The loads carry no branches: rows past R (or columns past OPAD) are
+    // clamped onto the last valid row and feed only output rows/columns that
+    // are never stored; k past K is clamped onto the row's last VA elements,
+    // finite values that meet the zero padding of Ct.
+    // parts [S0, S1) of the A tile; the MFMA loop stores and reloads it in two halves
+    auto load_a = [&](int k0, auto S0, auto S1) {
+        static_for<decltype(S0)::value, decltype(S1)::value>([&](auto S) {
+            constexpr int s = decltype(S)::value;
             const int idx = tid + kGThreads * s;
             const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
-            const int64_t gr = r0 + row;
-            const int gk = k0 + kc;
-            if (gr < R && gk < K) {
-                ra[s] = load_vec<float, VA>(A + gr * K + gk);
-            } else {
-#pragma unroll
-                for (int v = 0; v < VA; ++v) ra[s].v[v] = 0.f;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < B_LD; ++s) {
-            const int idx = tid + kGThreads * s;
-            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
-            const int gc = c0 + row;
-            if (gc < OPAD) {
-                rbh[s] = *reinterpret_cast<const uint4*>(Cth + (int64_t)gc * KP + k0 + kc);
-                rbl[s] = *reinterpret_cast<const uint4*>(Ctl + (int64_t)gc * KP + k0 + kc);
-            } else {
-                rbh[s] = make_uint4(0, 0, 0, 0);
-                rbl[s] = make_uint4(0, 0, 0, 0);
-            }
-        }
+            const int64_t gr = min(r0 + row, R - 1);
+            const int gk = min(k0 + kc, K - VA);
+            ra[s] = load_vec<float, VA>(A + gr * K + gk);
+        });
     };
-    auto store_tiles = [&](int stg) {
-#pragma unroll
-        for (int s = 0; s < A_LD; ++s) {
+    auto load_b = [&](auto q, int k0) {
+        const int idx = tid + kGThreads * q;
+        const int row = idx >> 2, kc = (idx & 3) * 8;
+        const int64_t o = (int64_t)min(c0 + row, OPAD - 1) * KP + k0 + kc;
+        rbh[q] = *reinterpret_cast<const u32x4*>(Cth + o);
+        rbl[q] = *reinterpret_cast<const u32x4*>(Ctl + o);
+    };
+    auto store_a = [&](int stg, auto S0, auto S1) {
+        static_for<decltype(S0)::value, decltype(S1)::value>([&](auto S) {
+            constexpr int s = decltype(S)::value;
             const int idx = tid + kGThreads * s;
             const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
+            const int e = swz16(row, kc >> 3) + (kc & 7);
 #pragma unroll
-            for (int v = 0; v < VA; ++v)
-                split_bf16(ra[s].v[v], Ah[stg][row][kc + v], Al[stg][row][kc + v]);
-        }
-#pragma unroll
-        for (int s = 0; s < B_LD; ++s) {
-            const int idx = tid + kGThreads * s;
-            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
-            *reinterpret_cast<uint4*>(&Bh[stg][row][kc]) = rbh[s];
-            *reinterpret_cast<uint4*>(&Bl[stg][row][kc]) = rbl[s];
-        }
+            for (int v = 0; v < VA; ++v) split_bf16(ra[s].v[v], Ah[stg][row][e + v], Al[stg][row][e + v]);
+        });
+    };
+    using A0 = std::integral_constant<int, 0>;
+    using AH = std::integral_constant<int, (NBW > 2 ? A_LD / 2 : A_LD)>;
+    using AE = std::integral_constant<int, A_LD>;
+    auto store_b = [&](auto q, int stg) {
+        const int idx = tid + kGThreads * q;
+        const int row = idx >> 2, e = swz16(row, idx & 3);
+        *reinterpret_cast<u32x4*>(&Bh[stg][row][e]) = rbh[q];
+        *reinterpret_cast<u32x4*>(&Bl[stg][row][e]) = rbl[q];
     };
 
     fx4 acc[4][NBW];
@@ -788,37 +819,72 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
         for (int j = 0; j < NBW; ++j) acc[i][j] = fx4{0, 0, 0, 0};
 
     const int nk = KP / kBK16;
-    load_tiles(0);
-    store_tiles(0);
+    load_a(0, A0{}, AE{});
+#pragma unroll
+    for (int q = 0; q < B_LD; ++q) load_b(q, 0);
+    store_a(0, A0{}, AE{});
+#pragma unroll
+    for (int q = 0; q < B_LD; ++q) store_b(q, 0);
     __syncthreads();
-    if (nk > 1) load_tiles(kBK16);
+    {
+        const int k1 = (nk > 1 ? 1 : 0) * kBK16;
+        load_a(k1, A0{}, AE{});
+#pragma unroll
+        for (int q = 0; q < B_LD; ++q) load_b(q, k1);
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) pin(ra[s]);
+#pragma unroll
+        for (int q = 0; q < B_LD; ++q) {
+        }
+    }
+    const int fr = lane & 15, fc = lane >> 4;  // fragment row / 16-byte chunk of this lane
     for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        bf16x8 ah[4], al[4];
+        const int cur = kt & 1, nxt = cur ^ 1;
+        // registers hold step kt+1 (loaded one step ago); they are stored to the
+        // other stage and refilled with step kt+2 (a clamped repeat near the end)
+        const int kpre = min(kt + 2, nk - 1) * kBK16;
+        bf16x8 ah[4], al[4], bh[2], bl[2];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            const int r = wr * 64 + i * 16 + fr;
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
         }
-        // Every block is computed, also those past OPAD (their B rows are zero),
-        // so the MFMA sequence has no branches and the accumulators stay in
-        // place; the three products of a block are issued in separate sweeps
-        // so that consecutive MFMAs never depend on each other.
-#pragma unroll
-        for (int j = 0; j < NBW; ++j) {
-            const int br = wc * 16 * NBW + j * 16 + (lane & 15);
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&Bh[cur][br][8 * (lane >> 4)]);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&Bl[cur][br][8 * (lane >> 4)]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+        {
+            const int r = wc * 16 * NBW + fr;
+            bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
+            bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
         }
-        // the other stage was last read before the previous barrier
-        if (kt + 1 < nk) store_tiles(cur ^ 1);
-        if (kt + 2 < nk) load_tiles((kt + 2) * kBK16);
+        // One group per 16-column block j: the fragments of block j+1 are read
+        // while the 12 MFMAs of block j run (three sweeps, so consecutive MFMAs
+        // never depend on each other), and the staging work of the next steps
+        // sits in fixed groups so that it issues under the MFMAs.  Every block
+        // is computed, also those past OPAD (their B rows are clamped copies).
+        static_for<0, NBW>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            if constexpr (j + 1 < NBW) {
+                const int r = wc * 16 * NBW + (j + 1) * 16 + fr;
+                bh[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
+                bl[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
+            }
+            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(al[I], bh[j & 1], acc[I][j]); });
+            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bl[j & 1], acc[I][j]); });
+            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bh[j & 1], acc[I][j]); });
+            // the other stage was last read before the previous barrier
+            if constexpr (j == 0) {
+                store_a(nxt, A0{}, AH{});
+                load_a(kpre, A0{}, AH{});
+            }
+            if constexpr (j == 2 && AH::value < A_LD) {
+                store_a(nxt, AH{}, AE{});
+                load_a(kpre, AH{}, AE{});
+            }
+            if constexpr ((j & 1) && (j >> 1) < B_LD) {
+                store_b(std::integral_constant<int, (j >> 1)>{}, nxt);
+                load_b(std::integral_constant<int, (j >> 1)>{}, kpre);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
         __syncthreads();
     }
 #pragma unroll
@@ -853,10 +919,10 @@ __global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
 __device__ __forceinline__ void ks_bf16x3_kernel_body(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
     int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate, int b, int nblocks) {
-    __shared__ __bf16 Ah[2][kTile][kPitch16];
-    __shared__ __bf16 Al[2][kTile][kPitch16];
-    __shared__ __bf16 Bh[2][kTile][kPitch16];
-    __shared__ __bf16 Bl[2][kTile][kPitch16];
+    __shared__ __bf16 Ah[2][kTile][kBK16];
+    __shared__ __bf16 Al[2][kTile][kBK16];
+    __shared__ __bf16 Bh[2][kTile][kBK16];
+    __shared__ __bf16 Bl[2][kTile][kBK16];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -872,41 +938,48 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
     const int Pb = (int)(((int64_t)Pc * (s + 1)) / nsplit);
 
     constexpr int LD = (kTile * kBK16 / 8) / kGThreads;  // 16-byte loads per array (= 2)
-    uint4 rah[LD], ral[LD], rbh[LD], rbl[LD];
+    u32x4 rah[LD], ral[LD], rbh[LD], rbl[LD];
 
-    auto load_tiles = [&](int P, int i0) {
+    // Branch-free loads: rows past N are clamped onto row N-1 and feed only
+    // rows/columns of the tile that are never stored.
+    auto load_a = [&](int P, int i0) {
         const int64_t base = (int64_t)P * N * OPAD + ioff + i0;
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
             const int idx = tid + kGThreads * q;
-            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
-            if (m0 + row < N) {
-                const int64_t o = base + (int64_t)(m0 + row) * OPAD + kc;
-                rah[q] = *reinterpret_cast<const uint4*>(Xh + o);
-                ral[q] = *reinterpret_cast<const uint4*>(Xl + o);
-            } else {
-                rah[q] = make_uint4(0, 0, 0, 0);
-                ral[q] = make_uint4(0, 0, 0, 0);
-            }
-            if (n0 + row < N) {
-                const int64_t o = base + (int64_t)(n0 + row) * OPAD + kc;
-                rbh[q] = *reinterpret_cast<const uint4*>(Xh + o);
-                rbl[q] = *reinterpret_cast<const uint4*>(Xl + o);
-            } else {
-                rbh[q] = make_uint4(0, 0, 0, 0);
-                rbl[q] = make_uint4(0, 0, 0, 0);
-            }
+            const int row = idx >> 2, kc = (idx & 3) * 8;
+            const int64_t o = base + (int64_t)min(m0 + row, N - 1) * OPAD + kc;
+            rah[q] = *reinterpret_cast<const u32x4*>(Xh + o);
+            ral[q] = *reinterpret_cast<const u32x4*>(Xl + o);
         }
     };
-    auto store_tiles = [&](int stg) {
+    auto load_b = [&](int P, int i0) {
+        const int64_t base = (int64_t)P * N * OPAD + ioff + i0;
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
             const int idx = tid + kGThreads * q;
-            const int row = idx / (kBK16 / 8), kc = (idx % (kBK16 / 8)) * 8;
-            *reinterpret_cast<uint4*>(&Ah[stg][row][kc]) = rah[q];
-            *reinterpret_cast<uint4*>(&Al[stg][row][kc]) = ral[q];
-            *reinterpret_cast<uint4*>(&Bh[stg][row][kc]) = rbh[q];
-            *reinterpret_cast<uint4*>(&Bl[stg][row][kc]) = rbl[q];
+            const int row = idx >> 2, kc = (idx & 3) * 8;
+            const int64_t o = base + (int64_t)min(n0 + row, N - 1) * OPAD + kc;
+            rbh[q] = *reinterpret_cast<const u32x4*>(Xh + o);
+            rbl[q] = *reinterpret_cast<const u32x4*>(Xl + o);
+        }
+    };
+    auto store_a = [&](int stg) {
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            const int idx = tid + kGThreads * q;
+            const int row = idx >> 2, e = swz16(row, idx & 3);
+            *reinterpret_cast<u32x4*>(&Ah[stg][row][e]) = rah[q];
+            *reinterpret_cast<u32x4*>(&Al[stg][row][e]) = ral[q];
+        }
+    };
+    auto store_b = [&](int stg) {
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            const int idx = tid + kGThreads * q;
+            const int row = idx >> 2, e = swz16(row, idx & 3);
+            *reinterpret_cast<u32x4*>(&Bh[stg][row][e]) = rbh[q];
+            *reinterpret_cast<u32x4*>(&Bl[stg][row][e]) = rbl[q];
         }
     };
 
@@ -922,47 +995,67 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
 
     const int nkp = width / kBK16;
     // steps are numbered across P; step st+1's tiles are written to the other
-    // LDS stage while the MFMAs of step st run
+    // LDS stage while the MFMAs of step st run, and step st+2 is loaded
     const int nsteps = (Pb - Pa) * nkp;
-    auto load_step = [&](int st) { load_tiles(Pa + st / nkp, (st % nkp) * kBK16); };
     if (nsteps > 0) {
-        load_step(0);
-        store_tiles(0);
+        load_a(Pa, 0);
+        load_b(Pa, 0);
+        store_a(0);
+        store_b(0);
+        const int s1 = nsteps > 1 ? 1 : 0;
+        load_a(Pa + s1 / nkp, (s1 % nkp) * kBK16);
+        load_b(Pa + s1 / nkp, (s1 % nkp) * kBK16);
+#pragma unroll
+        for (int q = 0; q < LD; ++q) {
+            pin(rah[q]);
+            pin(ral[q]);
+            pin(rbh[q]);
+            pin(rbl[q]);
+        }
     }
     __syncthreads();
-    if (nsteps > 1) load_step(1);
+    const int fr = lane & 15, fc = lane >> 4;  // fragment row / 16-byte chunk of this lane
     int st = 0;
     for (int P = Pa; P < Pb; ++P) {
     for (int kt = 0; kt < nkp; ++kt, ++st) {
-        const int cur = st & 1;
-        bf16x8 ah[4], al[4];
+        const int cur = st & 1, nxt = cur ^ 1;
+        const int sp = min(st + 2, nsteps - 1);  // clamped prefetch: the last loads are harmless repeats
+        const int Ppre = Pa + sp / nkp, ipre = (sp % nkp) * kBK16;
+        bf16x8 ah[4], al[4], bh[2], bl[2];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][wr * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+            const int r = wr * 64 + i * 16 + fr;
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
         }
-        bf16x8 bh[4], bl[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int br = wc * 64 + j * 16 + (lane & 15);
-            bh[j] = *reinterpret_cast<const bf16x8*>(&Bh[cur][br][8 * (lane >> 4)]);
-            bl[j] = *reinterpret_cast<const bf16x8*>(&Bl[cur][br][8 * (lane >> 4)]);
+        {
+            const int r = wc * 64 + fr;
+            bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
+            bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
         }
-        // lo*hi, hi*lo, then hi*hi: each sweep touches all 16 accumulators once
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
-        if (st + 1 < nsteps) store_tiles(cur ^ 1);
-        if (st + 2 < nsteps) load_step(st + 2);
+        // one group per 16-column block j (as in the half-transform): block
+        // j+1's fragments are read and one staging task is issued under the
+        // 12 MFMAs of block j
+        static_for<0, 4>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            if constexpr (j + 1 < 4) {
+                const int r = wc * 64 + (j + 1) * 16 + fr;
+                bh[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
+                bl[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
+            }
+            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(al[I], bh[j & 1], acc[I][j]); });
+            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bl[j & 1], acc[I][j]); });
+            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bh[j & 1], acc[I][j]); });
+            if constexpr (j == 0) {
+                store_a(nxt);
+                load_a(Ppre, ipre);
+            }
+            if constexpr (j == 1) {
+                store_b(nxt);
+                load_b(Ppre, ipre);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
         __syncthreads();
     }
     // end of one P: fold the FP32 partials into FP64
