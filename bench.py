@@ -46,6 +46,16 @@ def parse():
     ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 4 GiB device copy (x3), the reference for HBM byte counters")
+    ap.add_argument("--serve", choices=["water", "alkane"], default=None,
+                    help="instead of the J/K microbenchmark, serve a stream of SCF requests and report "
+                         "p50/p99 latency and jobs/s: (H2O)_n RHF cc-pVDZ, or C_nH_(2n+2) (+ cations) UHF cc-pVTZ")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-nmax", type=int, default=None, help="largest cluster/chain (24 water, 20 alkane)")
+    ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--serve-slots", type=int, default=12, help="jobs in flight")
+    ap.add_argument("--serve-prep-threads", type=int, default=2,
+                    help="host threads preparing admitted jobs (integral threads = cores / this)")
     return ap.parse_args()
 
 
@@ -103,6 +113,72 @@ def scf_mode(args):
     print(json.dumps(out))
 
 
+def host_cores():
+    """Host threads this process may use: OMP_NUM_THREADS when set (the share of
+    the host given to one GPU), else the affinity mask."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+    except ValueError:
+        return n
+
+
+def serve_mode(args):
+    """A stream of SCF requests through the continuous-batching server."""
+    import numpy as np
+    import torch
+
+    from forte2_amd import integrals
+    from forte2_amd.serving import SCFServer, alkane_stream, serving_stream
+
+    if args.serve == "water":
+        nmax = args.serve_nmax or 24
+        reqs = serving_stream(nreq=args.serve_requests, rate=args.serve_rate, nmax=nmax)
+        workload = (f"{len(reqs)} RHF requests, (H2O)_n n~U{{1..{nmax}}}, cc-pVDZ/cc-pVTZ-JKFIT, "
+                    f"Poisson arrivals at {args.serve_rate}/s (seed 7)")
+    else:
+        nmax = args.serve_nmax or 20
+        reqs = alkane_stream(nreq=args.serve_requests, rate=args.serve_rate, nmax=nmax)
+        ncat = sum(1 for r in reqs if r.charge == 1)
+        workload = (f"{len(reqs)} UHF requests, C_nH_(2n+2) n~U{{2..{nmax}}} ({ncat} cations, doublet; "
+                    f"neutrals ms=0), cc-pVTZ/cc-pVTZ-JKFIT, Poisson arrivals at {args.serve_rate}/s (seed 11)")
+    cores = host_cores()
+    ithreads = max(1, cores // args.serve_prep_threads)
+    integrals.set_threads(ithreads)
+    last = [0.0]
+
+    def log(msg):
+        t = time.perf_counter()
+        if t - last[0] >= 10.0:
+            last[0] = t
+            print(f"[serve] {msg}", file=sys.stderr, flush=True)
+
+    print(f"[serve] {workload}; host cores {cores}, {args.serve_prep_threads} prep threads x "
+          f"{ithreads} integral threads", file=sys.stderr, flush=True)
+    srv = SCFServer(max_active=args.serve_slots, prep_threads=args.serve_prep_threads,
+                    precision=args.serve_precision, log=log)
+    res = srv.run(reqs)
+    torch.cuda.synchronize()
+    s = srv.summary()
+    lat = sorted(r.latency for r in res.values() if r.error is None and r.converged)
+    queue = [r.admitted - r.arrival for r in res.values() if r.error is None]
+    gpu = [r.finished - r.started for r in res.values() if r.error is None and r.started > 0]
+    fails = {r.id: r.error for r in res.values() if r.error is not None or not r.converged}
+    print(json.dumps({
+        "metric": "SCF serving latency p50/p99 and jobs/s",
+        "value": s["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True,
+        "latency_p50_s": s["latency_p50_s"], "latency_p99_s": s["latency_p99_s"],
+        "latency_mean_s": s["latency_mean_s"], "latency_max_s": lat[-1] if lat else None,
+        "queue_wait_mean_s": float(np.mean(queue)) if queue else None,
+        "device_residence_mean_s": float(np.mean(gpu)) if gpu else None,
+        "requests": s["requests"], "completed": s["completed"], "failed": s["failed"],
+        "failures": dict(list(fails.items())[:5]), "wall_s": s["wall_s"], "ticks": s["ticks"],
+        "mean_group_size": s["mean_group_size"], "precision": args.serve_precision,
+        "config": {"workload": workload, "slots": args.serve_slots,
+                   "prep_threads": args.serve_prep_threads, "integral_threads": ithreads},
+    }))
+
+
 def timed_builds(fb, C, prec, steps, warmup):
     import torch
 
@@ -144,7 +220,7 @@ def cpu_baseline(B, C, q_sample):
 
     from oracle import jk as ojk
 
-    cores = len(os.sched_getaffinity(0))
+    cores = host_cores()
     Bs = B[:q_sample].cpu().numpy()
     Ch = C.cpu().numpy()
     with threadpool_limits(limits=cores):
@@ -176,6 +252,9 @@ def main():
         sys.exit(2)
     if args.scf:
         scf_mode(args)
+        return
+    if args.serve:
+        serve_mode(args)
         return
     if args.copy_calibration:
         copy_calibration()

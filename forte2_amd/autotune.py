@@ -1,41 +1,71 @@
-"""Launch-shape autotuner for the J/K kernels.
+"""Launch-shape autotuner for the J/K kernels, single-job and grouped.
 
 The tile edges of the kernels are fixed by their register/LDS layout (128x128
-FP64/FP32 tiles, 128 x 32*NBW BF16x3 tiles with NBW chosen from nocc).  What
-varies with the job size is the launch shape:
-  * ``target_blocks``: how many workgroups a split launch aims for.  This
-    fixes the P chunks of J pass 2 and the P splits of the K contraction,
-    i.e. the parallelism / partial-buffer trade-off;
+FP64/FP32 tiles, 128 x 32*NBW BF16x3 tiles).  What varies with the job size is
+the launch shape:
+  * ``target_blocks``: how many workgroups a split launch aims for.  It fixes
+    the P chunks of J pass 2 and the P splits of the K contraction, i.e. the
+    parallelism / partial-buffer trade-off;
   * ``xchunk_bytes``: the size of the half-transformed chunk X kept in the
-    workspace.  It trades L2/MALL reuse between KX and KS against launch
-    count.
+    workspace (single-job builds; a grouped build holds X in one chunk).  It
+    trades L2/MALL reuse between KX and KS against launch count;
+  * ``bf16_nbw_max`` (BF16x3): the widest half-transform tile, 32*NBW
+    columns.  Narrower tiles use fewer registers and LDS but re-read the B
+    rows once per column tile.
 
 ``tune(shapes)`` times every candidate on synthetic B of each shape with HIP
-events and keeps the fastest; ``save``/``load`` keep the table as JSON
-(``forte2_amd/data/tuning_gfx950.json`` ships the table measured on an
-MI355X).  ``apply(fb)`` sets the entry of the nearest tuned size class on a
-FockBuilder.  Candidates change only the decomposition: every candidate gives
-the same J/K up to the reduction order (parity is tested separately), and a
-given table gives bitwise-stable results.
+events and keeps the fastest; ``tune_grouped(shapes)`` does the same for a
+grouped build of several jobs of one shape, timed through the serving layer's
+group path (the shapes the server batches).  ``save``/``load`` keep the table
+as JSON (``forte2_amd/data/tuning_gfx950.json`` ships the table measured on an
+MI355X).  ``apply(fb, precision, grouped=...)`` sets the entry of the nearest
+tuned size class on a FockBuilder.  Candidates change only the decomposition:
+every candidate gives the same J/K up to the reduction order (parity is tested
+separately), and a given table gives bitwise-stable results.
+
+Table keys: "{precision}:{nbf}:{naux}" (single-job) and
+"group:{precision}:{nbf}:{naux}" (grouped).
 """
 
+import itertools
 import json
+import math
 import os
+import time
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning_gfx950.json")
 
 CANDIDATES = {
     "target_blocks": [256, 512, 1024, 2048, 4096],
     "xchunk_bytes": [1 << 29, 1 << 30, 1 << 31, 1 << 32],
+    "bf16_nbw_max": [4, 6, 8, 10],
 }
+KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max")
 
 
-def _key(nbf, naux, precision):
-    return f"{precision}:{nbf}:{naux}"
+def _key(nbf, naux, precision, grouped=False):
+    return ("group:" if grouped else "") + f"{precision}:{nbf}:{naux}"
+
+
+def _noccs(o):
+    return list(o) if isinstance(o, (list, tuple)) else [int(o)]
+
+
+def _candidates(precision, grouped):
+    names = ["target_blocks"] + ([] if grouped else ["xchunk_bytes"])
+    if precision == "bf16x3":
+        names.append("bf16_nbw_max")
+    for vals in itertools.product(*[CANDIDATES[n] for n in names]):
+        yield dict(zip(names, vals))
+
+
+def _set(fb, cand):
+    for k, v in cand.items():
+        fb.set_tuning(k, v)
 
 
 def tune(shapes, precisions=("fp64",), reps=3, device=None, log=print):
-    """shapes: [(nbf, naux, nocc)]; returns {key: {"target_blocks", "xchunk_bytes", "ms"}}."""
+    """shapes: [(nbf, naux, nocc or [nocc per density])]; returns {key: entry}."""
     import torch
 
     from .jkbuilder import FockBuilder
@@ -45,28 +75,79 @@ def tune(shapes, precisions=("fp64",), reps=3, device=None, log=print):
     table = {}
     for N, Q, o in shapes:
         B = synthetic_B(Q, N, seed=N + Q, device=dev)
-        C = orthonormal_C(N, o, seed=N, device=dev)
+        Cs = [orthonormal_C(N, x, seed=N + k, device=dev) for k, x in enumerate(_noccs(o))]
+        fb = FockBuilder(B=B, device=dev)
         for prec in precisions:
             best = None
-            for tb in CANDIDATES["target_blocks"]:
-                for xb in CANDIDATES["xchunk_bytes"]:
-                    fb = FockBuilder(B=B, device=dev)
-                    fb.set_tuning("target_blocks", tb)
-                    fb.set_tuning("xchunk_bytes", xb)
-                    fb.build_JK([C], precision=prec)  # warm-up (allocates the workspace)
-                    fb.timing = True
-                    ms = []
-                    for _ in range(reps):
-                        fb.build_JK([C], precision=prec)
-                        ms.append(fb.last_timings["total"])
-                    t = sorted(ms)[len(ms) // 2]
-                    if best is None or t < best["ms"]:
-                        best = {"target_blocks": tb, "xchunk_bytes": xb, "ms": t}
-                    del fb
+            for cand in _candidates(prec, grouped=False):
+                _set(fb, cand)
+                fb.timing = False
+                fb.build_JK(Cs, precision=prec)  # warm-up (workspace of this plan)
+                fb.timing = True
+                ms = []
+                for _ in range(reps):
+                    fb.build_JK(Cs, precision=prec)
+                    ms.append(fb.last_timings["total"])
+                t = sorted(ms)[len(ms) // 2]
+                if best is None or t < best["ms"]:
+                    best = dict(cand, ms=t)
+            fb.timing = False
             table[_key(N, Q, prec)] = best
             if log:
                 log(f"[autotune] N={N} Q={Q} o={o} {prec}: {best}")
-        del B
+        del fb, B
+        torch.cuda.empty_cache()
+    return table
+
+
+def tune_grouped(shapes, precisions=("fp64",), group=4, reps=3, device=None, log=print,
+                 hbm_fraction=0.6):
+    """Grouped builds of `group` jobs of each shape (fewer when their B would
+    not fit `hbm_fraction` of the free HBM); returns {"group:...": entry}."""
+    import torch
+
+    from .jkbuilder import FockBuilder
+    from .serving import SCFServer
+    from .workloads import orthonormal_C, synthetic_B
+
+    class _Pending:
+        def __init__(self, fb, C, prec):
+            self.fb, self.pending = fb, ("JK", C, prec)
+
+    dev = torch.device(device) if device is not None else torch.device("cuda", 0)
+    srv = SCFServer(device=dev)
+    table = {}
+    for N, Q, o in shapes:
+        if N % 2:
+            continue  # the grouped path needs an even nbf
+        free, _ = torch.cuda.mem_get_info(dev)
+        per_job = 12 * Q * N * N + 8 * Q * N * 64 * len(_noccs(o))
+        g = max(1, min(group, int(hbm_fraction * free // per_job)))
+        fbs, Cs = [], []
+        for j in range(g):
+            fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=N + Q + j, device=dev), device=dev))
+            Cs.append([orthonormal_C(N, x, seed=N + 7 * j + k, device=dev) for k, x in enumerate(_noccs(o))])
+        for prec in precisions:
+            best = None
+            for cand in _candidates(prec, grouped=True):
+                for fb in fbs:
+                    _set(fb, cand)
+                jobs = [_Pending(fb, C, prec) for fb, C in zip(fbs, Cs)]
+                srv._build_group(jobs)  # warm-up
+                torch.cuda.synchronize(dev)
+                ms = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    srv._build_group(jobs)
+                    torch.cuda.synchronize(dev)
+                    ms.append(1e3 * (time.perf_counter() - t0))
+                t = sorted(ms)[len(ms) // 2]
+                if best is None or t < best["ms"]:
+                    best = dict(cand, ms=t, group=g)
+            table[_key(N, Q, prec, grouped=True)] = best
+            if log:
+                log(f"[autotune] group of {g}: N={N} Q={Q} o={o} {prec}: {best}")
+        del fbs, Cs
         torch.cuda.empty_cache()
     return table
 
@@ -83,14 +164,15 @@ def load(path=TABLE):
         return json.load(f)
 
 
-def lookup(nbf, naux, precision="fp64", table=None):
+def lookup(nbf, naux, precision="fp64", table=None, grouped=False):
     """Entry of the tuned size class nearest to (nbf, naux) (log-distance), or None."""
-    import math
-
     table = load() if table is None else table
     best, bd = None, None
     for k, v in table.items():
-        prec, n, q = k.split(":")
+        parts = k.split(":")
+        if (parts[0] == "group") != grouped:
+            continue
+        prec, n, q = parts[-3:]
         if prec != precision:
             continue
         d = abs(math.log(int(n) / nbf)) + abs(math.log(int(q) / naux))
@@ -99,13 +181,25 @@ def lookup(nbf, naux, precision="fp64", table=None):
     return best
 
 
-def apply(fb, precision="fp64", table=None):
+def apply(fb, precision="fp64", table=None, grouped=False):
     """Set the tuned launch shape of the nearest size class on FockBuilder `fb`."""
-    e = lookup(fb.nbf, fb.naux, precision, table)
+    e = lookup(fb.nbf, fb.naux, precision, table, grouped)
     if e is not None:
-        fb.set_tuning("target_blocks", e["target_blocks"])
-        fb.set_tuning("xchunk_bytes", e["xchunk_bytes"])
+        for k in KEYS:
+            if k in e:
+                fb.set_tuning(k, e[k])
     return e
+
+
+def water_shapes(ns):
+    """(nbf, naux, nocc) of (H2O)_n in cc-pVDZ / cc-pVTZ-JKFIT."""
+    return [(24 * n, 139 * n, 5 * n) for n in ns]
+
+
+def alkane_shapes(ns):
+    """(nbf, naux, [nocc_alpha, nocc_beta]) of the C_nH_(2n+2) cations in
+    cc-pVTZ / cc-pVTZ-JKFIT (two densities per build)."""
+    return [(58 * n + 28, 139 * n + 60, [4 * n + 1, 4 * n]) for n in ns]
 
 
 if __name__ == "__main__":
@@ -113,8 +207,18 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="tune J/K launch shapes on this GPU")
     ap.add_argument("--waters", default="1,2,4,8,12,16,24,32,64")
+    ap.add_argument("--alkanes", default="2,5,10,15,20")
     ap.add_argument("--precisions", default="fp64,bf16x3")
+    ap.add_argument("--grouped-waters", default="2,4,8,12,16,24")
+    ap.add_argument("--group", type=int, default=4)
     ap.add_argument("--out", default=TABLE)
     a = ap.parse_args()
-    shapes = [(24 * n, 139 * n, 5 * n) for n in map(int, a.waters.split(","))]
-    save(tune(shapes, tuple(a.precisions.split(","))), a.out)
+    log = lambda m: print(m, flush=True)
+    precs = tuple(a.precisions.split(","))
+    ints = lambda s: [int(x) for x in s.split(",") if x]
+    table = {}
+    table.update(tune(water_shapes(ints(a.waters)) + alkane_shapes(ints(a.alkanes)), precs, log=log))
+    table.update(tune_grouped(water_shapes(ints(a.grouped_waters)) + alkane_shapes(ints(a.alkanes)), precs,
+                              group=a.group, log=log))
+    save(table, a.out)
+    log(f"[autotune] wrote {len(table)} entries to {a.out}")

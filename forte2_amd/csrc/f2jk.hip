@@ -122,7 +122,7 @@ int ensure_rowoff(f2jk_ctx* ctx, int vec) {
 }
 
 int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
-              bool j_from_c, Plan& p) {
+              bool j_from_c, Plan& p, int jvec_override = 0) {
     if (ctx->nbf <= 0 || ctx->naux <= 0) return fail(F2JK_ESTATE, "B has not been set");
     if (ndens < 1 || ndens > kMaxDens)
         return fail(F2JK_EINVAL, "ndens must be in [1, " + std::to_string(kMaxDens) + "]");
@@ -171,7 +171,7 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
     }
     if (p.want_j) {
         p.jb = (jdtype == F2JK_F64) ? 8 : 4;
-        p.jvec = j_vec_for(jdtype, N);
+        p.jvec = jvec_override ? jvec_override : j_vec_for(jdtype, N);
         int64_t nvec = 0;
         for (int64_t m = 0; m < N; ++m) nvec += (m + 1 + p.jvec - 1) / p.jvec;
         p.nvec = (int)nvec;
@@ -548,15 +548,17 @@ int group_plan(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const int64_t
                int flags, std::vector<Plan>& plans) {
     if (njobs < 1) return fail(F2JK_EINVAL, "njobs must be >= 1");
     if (!ctxs || !ndens || !nocc) return fail(F2JK_EINVAL, "null argument");
-    if (precision != F2JK_PREC_FP64) return fail(F2JK_EUNSUPPORTED, "grouped builds are FP64");
-    if ((flags & (F2JK_WANT_J | F2JK_WANT_K)) != (F2JK_WANT_J | F2JK_WANT_K))
-        return fail(F2JK_EUNSUPPORTED, "grouped builds compute J and K");
+    if (precision < F2JK_PREC_FP64 || precision > F2JK_PREC_BF16X3) return fail(F2JK_EINVAL, "unknown precision");
+    const int jk = flags & (F2JK_WANT_J | F2JK_WANT_K);
+    if (jk != (F2JK_WANT_J | F2JK_WANT_K) && jk != F2JK_WANT_K)
+        return fail(F2JK_EUNSUPPORTED, "grouped builds compute J and K, or K only");
     plans.resize(njobs);
     int64_t k = 0;
     for (int j = 0; j < njobs; ++j) {
         if (!ctxs[j]) return fail(F2JK_EINVAL, "null context in group");
         if (ndens[j] < 1 || ndens[j] > 2) return fail(F2JK_EUNSUPPORTED, "grouped jobs have 1 or 2 densities");
-        int rc = make_plan(ctxs[j], ndens[j], nocc + k, precision, flags, true, plans[j]);
+        // one J vector width (2) for the whole group, so one kernel serves every job
+        int rc = make_plan(ctxs[j], ndens[j], nocc + k, precision, flags, true, plans[j], 2);
         if (rc) return rc;
         const Plan& p = plans[j];
         if (p.N % 2) return fail(F2JK_EUNSUPPORTED, "grouped builds need an even nbf");
@@ -567,14 +569,59 @@ int group_plan(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const int64_t
     return F2JK_OK;
 }
 
-template <int PH>
+template <int PH, int PREC, int NBW = 2, int VA = 2>
 int launch_group_phase(const GArgs& a, hipStream_t st) {
     long long total = 0;
-    for (int j = 0; j < a.njobs; ++j) total += phase_blocks(a.job[j], PH);
+    for (int j = 0; j < a.njobs; ++j) total += phase_blocks<PREC>(a.job[j], PH);
     if (total == 0) return F2JK_OK;
     const unsigned grid = (unsigned)std::min<long long>(total, 4096);
-    hipLaunchKernelGGL(grouped_kernel<PH>, dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((grouped_kernel<PH, PREC, NBW, VA>), dim3(grid), dim3(256), 0, st, a);
     F2JK_LAUNCHED("grouped phase " + std::to_string(PH));
+    return F2JK_OK;
+}
+
+// the half-transform of a BF16x3 group: tile width NBW (the widest job's) and load width VA
+template <int VA>
+int launch_group_kx_bf16(const GArgs& a, int nbw, hipStream_t st) {
+    switch (nbw) {
+        case 2: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 2, VA>(a, st);
+        case 4: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 4, VA>(a, st);
+        case 6: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 6, VA>(a, st);
+        case 8: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 8, VA>(a, st);
+        default: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 10, VA>(a, st);
+    }
+}
+
+template <int PREC>
+int run_group(f2jk_ctx* ctx, const GArgs& a, int nbw, int va, hipStream_t st) {
+    int rc;
+    TIM(F2JK_PH_PREP, true);
+    if ((rc = launch_group_phase<GP_PAD, PREC>(a, st))) return rc;
+    if ((rc = launch_group_phase<GP_DKS, PREC>(a, st))) return rc;
+    if ((rc = launch_group_phase<GP_DRED, PREC>(a, st))) return rc;
+    TIM(F2JK_PH_PREP, false);
+    TIM(F2JK_PH_JGAMMA, true);
+    if ((rc = launch_group_phase<GP_GAMMA, PREC>(a, st))) return rc;
+    if ((rc = launch_group_phase<GP_GRED, PREC>(a, st))) return rc;
+    TIM(F2JK_PH_JGAMMA, false);
+    TIM(F2JK_PH_JACC, true);
+    if ((rc = launch_group_phase<GP_JACC, PREC>(a, st))) return rc;
+    if ((rc = launch_group_phase<GP_JRED, PREC>(a, st))) return rc;
+    TIM(F2JK_PH_JACC, false);
+    TIM(F2JK_PH_KX, true);
+    if constexpr (PREC == F2JK_PREC_BF16X3) {
+        rc = (va == 4) ? launch_group_kx_bf16<4>(a, nbw, st) : launch_group_kx_bf16<2>(a, nbw, st);
+        if (rc) return rc;
+    } else {
+        if ((rc = launch_group_phase<GP_KX, PREC>(a, st))) return rc;
+    }
+    TIM(F2JK_PH_KX, false);
+    TIM(F2JK_PH_KS, true);
+    if ((rc = launch_group_phase<GP_KS, PREC>(a, st))) return rc;
+    TIM(F2JK_PH_KS, false);
+    TIM(F2JK_PH_KRED, true);
+    if ((rc = launch_group_phase<GP_KRED, PREC>(a, st))) return rc;
+    TIM(F2JK_PH_KRED, false);
     return F2JK_OK;
 }
 
@@ -788,44 +835,61 @@ int f2jk_group_workspace_size(f2jk_ctx* const* ctxs, int njobs, const int* ndens
 int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const double* const* C,
                         const int64_t* nocc, const int64_t* ldc, double* const* J, double* const* K,
                         int precision, int flags, void* const* ws, const size_t* ws_bytes, void* stream) {
-    if (!C || !ldc || !J || !K || !ws || !ws_bytes) return fail(F2JK_EINVAL, "null argument");
+    if (!C || !ldc || !K || !ws || !ws_bytes) return fail(F2JK_EINVAL, "null argument");
     std::vector<Plan> plans;
     int rc = group_plan(ctxs, njobs, ndens, nocc, precision, flags, plans);
     if (rc) return rc;
+    const bool want_j = flags & F2JK_WANT_J;
+    if (want_j && !J) return fail(F2JK_EINVAL, "null argument");
     F2JK_HIP(hipSetDevice(ctxs[0]->device));
     hipStream_t st = (hipStream_t)stream;
+    // the group's BF16x3 tile width (widest job) and load width (4 when every nbf allows it)
+    int nbw = 2, va = 4;
+    for (int j = 0; j < njobs; ++j) {
+        nbw = std::max(nbw, plans[j].nbw);
+        if (plans[j].N % 4) va = 2;
+    }
     std::vector<GJob> jobs(njobs);
     int64_t k = 0;
     for (int j = 0; j < njobs; ++j) {
         const Plan& p = plans[j];
         if ((rc = check_ws(p, ws[j], ws_bytes[j]))) return rc;
-        if ((rc = ensure_rowoff(ctxs[j], p.jvec))) return rc;
+        if (want_j && (rc = ensure_rowoff(ctxs[j], p.jvec))) return rc;
         char* w = (char*)ws[j];
         GJob& g = jobs[j];
         std::memset(&g, 0, sizeof(g));
-        g.B = (const double*)ctxs[j]->B[F2JK_F64];
-        g.rowoff = ctxs[j]->rowoff[p.jvec];
+        g.B = ctxs[j]->B[precision == F2JK_PREC_FP64 ? F2JK_F64 : F2JK_F32];
+        g.rowoff = want_j ? ctxs[j]->rowoff[p.jvec] : nullptr;
         g.Cp = (double*)(w + p.off_cp);
-        g.D = (double*)(w + p.off_d);
-        g.Dpart = (double*)(w + p.off_dpart);
-        g.gpart = (double*)(w + p.off_gpart);
-        g.gamma = (double*)(w + p.off_gamma);
-        g.jpart = (double*)(w + p.off_jpart);
-        g.X = (double*)(w + p.off_x);
+        g.Cp32 = (float*)(w + p.off_cp32);
+        if (precision == F2JK_PREC_BF16X3) {
+            g.Cth = (u16*)(w + p.off_cth);
+            g.Ctl = (u16*)(w + p.off_ctl);
+        }
+        g.X = (void*)(w + p.off_x);
         g.Kpart = (double*)(w + p.off_kpart);
+        if (want_j) {
+            g.D = (double*)(w + p.off_d);
+            g.Dpart = (double*)(w + p.off_dpart);
+            g.gpart = (double*)(w + p.off_gpart);
+            g.gamma = (double*)(w + p.off_gamma);
+            g.jpart = (double*)(w + p.off_jpart);
+        }
+        g.want_j = want_j ? 1 : 0;
         g.N = p.N;
         g.Q = p.Q;
         g.nd = p.nd;
         g.OPAD = p.OPAD;
+        g.KP = p.KP;
         for (int d = 0; d < p.nd; ++d, ++k) {
-            if (!C[k] || !J[k] || !K[k]) return fail(F2JK_EINVAL, "null C/J/K pointer in group");
+            if (!C[k] || !K[k] || (want_j && !J[k])) return fail(F2JK_EINVAL, "null C/J/K pointer in group");
             if (nocc[k] > 0 && ldc[k] < nocc[k]) return fail(F2JK_EINVAL, "ldc < nocc");
             g.C[d] = C[k];
             g.ldc[d] = nocc[k] > 0 ? ldc[k] : 1;
             g.nocc[d] = (int)nocc[k];
             g.width[d] = p.width[d];
             g.ioff[d] = p.ioff[d];
-            g.J[d] = J[k];
+            g.J[d] = want_j ? J[k] : nullptr;
             g.K[d] = K[k];
         }
         g.nvec = p.nvec;
@@ -835,42 +899,26 @@ int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens, cons
         g.ppw2 = p.ppw2;
         g.nlower = p.nlower;
         g.nsplit = std::min(p.nsplit, std::min(p.Pc, p.Q));
-        g.ncolt = p.ncolt;
+        g.ncolt = (precision == F2JK_PREC_BF16X3) ? (int)cdiv(p.OPAD, 32 * nbw) : p.ncolt;
         // the D-build tiles use the same lower-tile count as the K tiles
         if (p.nlower != (int)(cdiv(p.N, kTile) * (cdiv(p.N, kTile) + 1) / 2))
             return fail(F2JK_EINVAL, "inconsistent tile plan");
     }
-    f2jk_ctx* tctx = ctxs[0];
-    tim_begin(tctx, flags & F2JK_TIMING);
-    f2jk_ctx* ctx = tctx;  // for TIM
+    f2jk_ctx* ctx = ctxs[0];  // timing events of the group
+    tim_begin(ctx, flags & F2JK_TIMING);
     TIM(F2JK_PH_TOTAL, true);
     for (int j0 = 0; j0 < njobs; j0 += kMaxGroup) {
         GArgs a;
         std::memset(&a, 0, sizeof(a));
         a.njobs = std::min(kMaxGroup, njobs - j0);
         for (int j = 0; j < a.njobs; ++j) a.job[j] = jobs[j0 + j];
-        TIM(F2JK_PH_PREP, true);
-        if ((rc = launch_group_phase<GP_PAD>(a, st))) return rc;
-        if ((rc = launch_group_phase<GP_DKS>(a, st))) return rc;
-        if ((rc = launch_group_phase<GP_DRED>(a, st))) return rc;
-        TIM(F2JK_PH_PREP, false);
-        TIM(F2JK_PH_JGAMMA, true);
-        if ((rc = launch_group_phase<GP_GAMMA>(a, st))) return rc;
-        if ((rc = launch_group_phase<GP_GRED>(a, st))) return rc;
-        TIM(F2JK_PH_JGAMMA, false);
-        TIM(F2JK_PH_JACC, true);
-        if ((rc = launch_group_phase<GP_JACC>(a, st))) return rc;
-        if ((rc = launch_group_phase<GP_JRED>(a, st))) return rc;
-        TIM(F2JK_PH_JACC, false);
-        TIM(F2JK_PH_KX, true);
-        if ((rc = launch_group_phase<GP_KX>(a, st))) return rc;
-        TIM(F2JK_PH_KX, false);
-        TIM(F2JK_PH_KS, true);
-        if ((rc = launch_group_phase<GP_KS>(a, st))) return rc;
-        TIM(F2JK_PH_KS, false);
-        TIM(F2JK_PH_KRED, true);
-        if ((rc = launch_group_phase<GP_KRED>(a, st))) return rc;
-        TIM(F2JK_PH_KRED, false);
+        if (precision == F2JK_PREC_FP64)
+            rc = run_group<F2JK_PREC_FP64>(ctx, a, nbw, va, st);
+        else if (precision == F2JK_PREC_FP32)
+            rc = run_group<F2JK_PREC_FP32>(ctx, a, nbw, va, st);
+        else
+            rc = run_group<F2JK_PREC_BF16X3>(ctx, a, nbw, va, st);
+        if (rc) return rc;
     }
     TIM(F2JK_PH_TOTAL, false);
     g_last_error.clear();

@@ -5,13 +5,23 @@
 // gridDim.x, finds its job from the per-phase block counts (a short uniform
 // scan, at most kMaxGroup jobs) and runs the same device body as the
 // single-job kernels with the job's own plan.  The decomposition of a job
-// (element groups, P chunks, K splits) is exactly its single-job plan, so a
-// job's J/K are bitwise the same alone and in any mix.
+// (element groups, P chunks, K splits) is its own plan, independent of the
+// other jobs; the tile width of the BF16x3 half-transform is the group's
+// (the widest job's), which changes which block computes an element of X but
+// not the order of its k-sum.  So a job's J/K are bitwise the same alone and
+// in any mix.
+//
+// Precisions: FP64 (J and K from the FP64 B), FP32 and BF16x3 (J from the
+// FP32 B with FP64 accumulation, K on MFMA from the FP32 B), one precision
+// per group.  A job computes J and K, or K only.
 //
 // The job table travels by value in the kernel arguments (kMaxGroup jobs fit
 // the 4 KiB argument limit), so a launch needs no descriptor upload.
-// Restrictions of the grouped path: FP64, nbf even, one X chunk per job.
+// Restrictions of the grouped path: nbf even, one X chunk per job.
 #pragma once
+#include <type_traits>
+
+#include "f2jk.h"
 #include "f2jk_kernels.h"
 
 namespace f2jk {
@@ -19,7 +29,8 @@ namespace f2jk {
 constexpr int kMaxGroup = 12;
 
 enum GroupPhase {
-    GP_PAD = 0,      // Cp[n][ioff+i] = C[n][i]
+    GP_PAD = 0,      // Cp[n][ioff+i] = C[n][i] (FP64; also the FP32 copy, or the
+                     // transposed hi/lo bf16 copies, of the precision forms)
     GP_DKS = 1,      // Dpart[d] = Cp_d Cp_d^T (lower tiles)
     GP_DRED = 2,     // D[d] mirrored from Dpart[d]
     GP_GAMMA = 3,    // J pass 1
@@ -33,35 +44,42 @@ enum GroupPhase {
 };
 
 struct GJob {
-    const double* B;
+    const void* B;        // FP64 B (FP64 groups) or the FP32 copy (FP32 / BF16x3 groups)
     const int* rowoff;
     const double* C[2];
-    double *Cp, *D, *Dpart, *gpart, *gamma, *jpart, *X, *Kpart;
+    double *Cp, *D, *Dpart, *gpart, *gamma, *jpart, *Kpart;
+    float* Cp32;          // FP32 coefficients (FP32 K)
+    u16 *Cth, *Ctl;       // transposed hi/lo coefficients (BF16x3 K)
+    void* X;              // half-transformed chunk: FP64, FP32, or hi then lo bf16
     double* J[2];
     double* K[2];
     long long ldc[2];
-    int N, Q, nd, nocc[2], width[2], ioff[2], OPAD;
+    int N, Q, nd, nocc[2], width[2], ioff[2], OPAD, KP;
     int nvec, neg, npc1, npc2, ppw2;
     int nlower, nsplit, ncolt;
+    int want_j;
 };
 
 struct GArgs {
     int njobs;
     GJob job[kMaxGroup];
 };
+static_assert(sizeof(GArgs) <= 4096, "the job table must fit the kernel argument limit");
 
 __host__ __device__ inline int gcdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // blocks of one job in one phase (256 threads per block everywhere)
+template <int PREC>
 __host__ __device__ inline int phase_blocks(const GJob& g, int phase) {
+    const int j = g.want_j ? 1 : 0;
     switch (phase) {
-        case GP_PAD: return gcdiv((long long)g.N * g.OPAD, 256);
-        case GP_DKS: return g.nlower * g.nd;
-        case GP_DRED: return g.N * gcdiv(g.N, 256) * g.nd;
-        case GP_GAMMA: return g.neg * g.npc1;
-        case GP_GRED: return gcdiv((long long)g.nd * g.Q, 256);
-        case GP_JACC: return g.neg * g.npc2;
-        case GP_JRED: return gcdiv(g.nvec, 256);
+        case GP_PAD: return gcdiv((long long)(PREC == F2JK_PREC_BF16X3 ? g.KP : g.N) * g.OPAD, 256);
+        case GP_DKS: return j * g.nlower * g.nd;
+        case GP_DRED: return j * g.N * gcdiv(g.N, 256) * g.nd;
+        case GP_GAMMA: return j * g.neg * g.npc1;
+        case GP_GRED: return j * gcdiv((long long)g.nd * g.Q, 256);
+        case GP_JACC: return j * g.neg * g.npc2;
+        case GP_JRED: return j * gcdiv(g.nvec, 256);
         case GP_KX: return gcdiv((long long)g.Q * g.N, kTile) * g.ncolt;
         case GP_KS: return g.nlower * g.nsplit * g.nd;
         case GP_KRED: return g.N * gcdiv(g.N, 256) * g.nd;
@@ -79,14 +97,17 @@ __device__ __forceinline__ void mirror_reduce(const double* part, int nsplit, in
     out[(long long)n * N + m] = acc;
 }
 
-template <int PHASE>
+// PREC: the group's precision; NBW, VA: tile width and load width of the
+// BF16x3 half-transform (unused otherwise).
+template <int PHASE, int PREC, int NBW = 2, int VA = 2>
 __global__ __launch_bounds__(256) void grouped_kernel(const GArgs a) {
+    using TB = typename std::conditional<PREC == F2JK_PREC_FP64, double, float>::type;
     int total = 0;
-    for (int j = 0; j < a.njobs; ++j) total += phase_blocks(a.job[j], PHASE);
+    for (int j = 0; j < a.njobs; ++j) total += phase_blocks<PREC>(a.job[j], PHASE);
     for (int lb = blockIdx.x; lb < total; lb += gridDim.x) {
         int j = 0, base = 0;
         for (;;) {
-            const int nb = phase_blocks(a.job[j], PHASE);
+            const int nb = phase_blocks<PREC>(a.job[j], PHASE);
             if (lb < base + nb) break;
             base += nb;
             ++j;
@@ -96,11 +117,27 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GArgs a) {
         const int tid = threadIdx.x;
         if constexpr (PHASE == GP_PAD) {
             const long long t = (long long)local * 256 + tid;
-            if (t < (long long)g.N * g.OPAD) {
+            if constexpr (PREC == F2JK_PREC_BF16X3) {
+                // one thread per (row n, column c) of Cp and per (c, k) of the
+                // transposed hi/lo copy, k < KP (zero past N)
+                if (t < (long long)g.KP * g.OPAD) {
+                    const int c = (int)(t / g.KP), k = (int)(t % g.KP);
+                    const int d = (g.nd == 2 && c >= g.ioff[1]) ? 1 : 0;
+                    const int i = c - g.ioff[d];
+                    const double x = (i < g.nocc[d] && k < g.N) ? g.C[d][(long long)k * g.ldc[d] + i] : 0.0;
+                    __bf16 h, l;
+                    split_bf16((float)x, h, l);
+                    g.Cth[t] = __builtin_bit_cast(u16, h);
+                    g.Ctl[t] = __builtin_bit_cast(u16, l);
+                    if (k < g.N) g.Cp[(long long)k * g.OPAD + c] = x;
+                }
+            } else if (t < (long long)g.N * g.OPAD) {
                 const int n = (int)(t / g.OPAD), c = (int)(t % g.OPAD);
                 const int d = (g.nd == 2 && c >= g.ioff[1]) ? 1 : 0;
                 const int i = c - g.ioff[d];
-                g.Cp[t] = (i < g.nocc[d]) ? g.C[d][(long long)n * g.ldc[d] + i] : 0.0;
+                const double x = (i < g.nocc[d]) ? g.C[d][(long long)n * g.ldc[d] + i] : 0.0;
+                g.Cp[t] = x;
+                if constexpr (PREC == F2JK_PREC_FP32) g.Cp32[t] = (float)x;
             }
         } else if constexpr (PHASE == GP_DKS) {
             const int d = local / g.nlower, b = local % g.nlower;
@@ -115,12 +152,13 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GArgs a) {
         } else if constexpr (PHASE == GP_GAMMA) {
             const int bx = local % g.neg, by = local / g.neg;
             const long long NN = (long long)g.N * g.N;
+            const TB* B = (const TB*)g.B;
             if (g.nd == 2) {
                 DPtrs<2> dp{{g.D, g.D + NN}};
-                j_gamma_kernel_body<double, 2, 4, 2>(g.B, g.N, g.Q, dp, g.N, g.rowoff, g.nvec, 32, g.gpart, bx, by);
+                j_gamma_kernel_body<TB, 2, 4, 2>(B, g.N, g.Q, dp, g.N, g.rowoff, g.nvec, 32, g.gpart, bx, by);
             } else {
                 DPtrs<1> dp{{g.D}};
-                j_gamma_kernel_body<double, 2, 4, 1>(g.B, g.N, g.Q, dp, g.N, g.rowoff, g.nvec, 32, g.gpart, bx, by);
+                j_gamma_kernel_body<TB, 2, 4, 1>(B, g.N, g.Q, dp, g.N, g.rowoff, g.nvec, 32, g.gpart, bx, by);
             }
         } else if constexpr (PHASE == GP_GRED) {
             const long long t = (long long)local * 256 + tid;
@@ -132,10 +170,11 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GArgs a) {
             }
         } else if constexpr (PHASE == GP_JACC) {
             const int bx = local % g.neg, by = local / g.neg;
+            const TB* B = (const TB*)g.B;
             if (g.nd == 2)
-                j_accum_kernel_body<double, 2, 4, 2>(g.B, g.N, g.Q, g.gamma, g.rowoff, g.nvec, g.ppw2, g.jpart, bx, by);
+                j_accum_kernel_body<TB, 2, 4, 2>(B, g.N, g.Q, g.gamma, g.rowoff, g.nvec, g.ppw2, g.jpart, bx, by);
             else
-                j_accum_kernel_body<double, 2, 4, 1>(g.B, g.N, g.Q, g.gamma, g.rowoff, g.nvec, g.ppw2, g.jpart, bx, by);
+                j_accum_kernel_body<TB, 2, 4, 1>(B, g.N, g.Q, g.gamma, g.rowoff, g.nvec, g.ppw2, g.jpart, bx, by);
         } else if constexpr (PHASE == GP_JRED) {
             const int e = local * 256 + tid;
             if (e < g.nvec) {
@@ -158,13 +197,29 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GArgs a) {
                 }
             }
         } else if constexpr (PHASE == GP_KX) {
-            const int nb = phase_blocks(g, GP_KX);
-            kx_kernel_body<double, 2>(g.B, (long long)g.Q * g.N, g.N, g.Cp, g.OPAD, g.X, g.ncolt, local, nb);
+            const int nb = phase_blocks<PREC>(g, GP_KX);
+            const long long R = (long long)g.Q * g.N;
+            if constexpr (PREC == F2JK_PREC_FP64) {
+                kx_kernel_body<double, 2>((const double*)g.B, R, g.N, g.Cp, g.OPAD, (double*)g.X, g.ncolt, local, nb);
+            } else if constexpr (PREC == F2JK_PREC_FP32) {
+                kx_kernel_body<float, 2>((const float*)g.B, R, g.N, g.Cp32, g.OPAD, (float*)g.X, g.ncolt, local, nb);
+            } else {
+                u16* xh = (u16*)g.X;
+                kx_bf16x3_kernel_body<NBW, VA>((const float*)g.B, R, g.N, g.KP, g.Cth, g.Ctl, g.OPAD, xh,
+                                               xh + R * g.OPAD, g.ncolt, local, nb);
+            }
         } else if constexpr (PHASE == GP_KS) {
             const int per = g.nlower * g.nsplit;
             const int d = local / per, b = local % per;
-            ks_kernel_body<double>(g.X, g.N, g.OPAD, g.ioff[d], g.width[d], g.Q, g.nsplit, g.nlower,
-                                   g.Kpart + (long long)d * g.nsplit * g.N * g.N, 0, b, per);
+            double* kp = g.Kpart + (long long)d * g.nsplit * g.N * g.N;
+            if constexpr (PREC == F2JK_PREC_BF16X3) {
+                const u16* xh = (const u16*)g.X;
+                ks_bf16x3_kernel_body(xh, xh + (long long)g.Q * g.N * g.OPAD, g.N, g.OPAD, g.ioff[d], g.width[d],
+                                      g.Q, g.nsplit, g.nlower, kp, 0, b, per);
+            } else {
+                ks_kernel_body<TB>((const TB*)g.X, g.N, g.OPAD, g.ioff[d], g.width[d], g.Q, g.nsplit, g.nlower,
+                                   kp, 0, b, per);
+            }
         } else if constexpr (PHASE == GP_KRED) {
             const int per = g.N * gcdiv(g.N, 256);
             const int d = local / per, r = local % per;

@@ -843,17 +843,21 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
         // registers hold step kt+1 (loaded one step ago); they are stored to the
         // other stage and refilled with step kt+2 (a clamped repeat near the end)
         const int kpre = min(kt + 2, nk - 1) * kBK16;
+        // fragment reads in the order the first group consumes them (bh, al for
+        // the first sweep, then bl, ah), so its first MFMA waits for two reads
         bf16x8 ah[4], al[4], bh[2], bl[2];
+        const int rb0 = wc * 16 * NBW + fr;
+        bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][rb0][swz16(rb0, fc)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = wr * 64 + i * 16 + fr;
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
+        }
+        bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][rb0][swz16(rb0, fc)]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = wr * 64 + i * 16 + fr;
             ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
-            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
-        }
-        {
-            const int r = wc * 16 * NBW + fr;
-            bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
-            bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
         }
         // One group per 16-column block j: the fragments of block j+1 are read
         // while the 12 MFMAs of block j run (three sweeps, so consecutive MFMAs
@@ -1021,17 +1025,20 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         const int cur = st & 1, nxt = cur ^ 1;
         const int sp = min(st + 2, nsteps - 1);  // clamped prefetch: the last loads are harmless repeats
         const int Ppre = Pa + sp / nkp, ipre = (sp % nkp) * kBK16;
+        // fragment reads in the order the first group consumes them
         bf16x8 ah[4], al[4], bh[2], bl[2];
+        const int rb0 = wc * 64 + fr;
+        bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][rb0][swz16(rb0, fc)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = wr * 64 + i * 16 + fr;
+            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
+        }
+        bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][rb0][swz16(rb0, fc)]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = wr * 64 + i * 16 + fr;
             ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
-            al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
-        }
-        {
-            const int r = wc * 64 + fr;
-            bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
-            bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
         }
         // one group per 16-column block j (as in the half-transform): block
         // j+1's fragments are read and one staging task is issued under the

@@ -5,7 +5,8 @@ server keeps each admitted job's B, density, Fock and DIIS history on the
 device.  It drives all active jobs in lock-step ticks.  At each tick, every
 job that needs a Fock matrix contributes its occupied orbitals to one grouped
 J/K build (``f2jk_group_build_jk``: one persistent launch per phase for up to
-12 jobs).  Then every job advances one SCF step on the device.
+12 jobs, in the job's precision: FP64, or the FP32 / BF16x3 precision form).
+Then every job advances one SCF step on the device.
 
 Admission is continuous:
   * a request becomes eligible at its arrival time;
@@ -73,11 +74,12 @@ class SCFResult:
         return self.finished - self.arrival
 
 
-def job_footprint(nbf, naux, nocc_total):
-    """HBM bytes a job holds: B plus an upper bound on its J/K workspace."""
+def job_footprint(nbf, naux, nocc_total, precision="fp64"):
+    """HBM bytes a job holds: B (plus its FP32 copy in the precision forms) and
+    an upper bound on its J/K workspace."""
     N, Q = nbf, naux
     opad = 32 * (int(nocc_total) // 16 + 2)
-    B = 8 * Q * N * N
+    B = (8 if precision == "fp64" else 12) * Q * N * N
     ws = 8 * Q * N * opad + 8 * N * N * (4 + 80) + 8 * 2048 * 4 * 1024 + (64 << 20)
     return B + ws
 
@@ -152,7 +154,8 @@ class SCFServer:
         system.device = self.device
         B = _df.device_B_from_host(M, T, self.device)
         fb = FockBuilder(system, B=B, device=self.device)
-        autotune.apply(fb, "fp64")  # launch shape of the job's size class
+        # launch shape of the job's size class (tuned for grouped builds when batching)
+        autotune.apply(fb, self.precision, grouped=self.grouped)
         system.fock_builder = fb
         kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter, precision=self.precision)
         scf = RHF(req.charge, **kw) if req.ms is None else UHF(req.charge, ms=req.ms, **kw)
@@ -161,57 +164,85 @@ class SCFServer:
         return _Job(req, system, scf, fb)
 
     # ------------------------------------------------------------------ grouped J/K
+    def _groupable(self, jb, kind, prec, nd):
+        """Whether the library takes this job into a grouped build (asked once per
+        kind/precision/density count: odd nbf, or an X larger than one chunk, run
+        the job on its own)."""
+        key = (kind, prec, nd)
+        memo = jb.__dict__.setdefault("_groupable", {})
+        if key not in memo:
+            lib = nat.jk_lib()
+            occ = jb.pending[1]
+            if prec != "fp64":
+                jb.fb.B_fp32()
+            flags = nat.F2JK_WANT_K | (nat.F2JK_WANT_J if kind == "JK" else 0)
+            sizes = (ctypes.c_size_t * 1)()
+            rc = lib.f2jk_group_workspace_size(nat.ptr_array([jb.fb.ctx.value]), 1, (ctypes.c_int * 1)(nd),
+                                               nat.i64_array([c.shape[1] for c in occ]), nat.PREC[prec],
+                                               flags, sizes)
+            memo[key] = rc == 0
+        return memo[key]
+
     def _build_group(self, jobs):
-        """J/K for every job in `jobs` (same density count); returns [(J, K)] per job."""
+        """The pending "JK" (or "K") builds of `jobs` (same kind, precision and
+        density count) in grouped launches; returns per job (J list, K list), or
+        the K list for "K"."""
         lib = nat.jk_lib()
+        kind, _, prec = jobs[0].pending
+        want_j = kind == "JK"
+        flags = nat.F2JK_WANT_K | (nat.F2JK_WANT_J if want_j else 0)
         ctxs, nds, Cs, nocc, ldc, Js, Ks = [], [], [], [], [], [], []
         outs = []
         for jb in jobs:
             occ = [c.contiguous() for c in jb.pending[1]]
             N = jb.fb.nbf
-            J = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ]
+            J = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ] if want_j else []
             K = [torch.empty((N, N), dtype=torch.float64, device=self.device) for _ in occ]
+            if prec != "fp64":
+                jb.fb.B_fp32()  # the FP32 copy the precision forms read
             jb._keep = occ
             ctxs.append(jb.fb.ctx.value)
             nds.append(len(occ))
-            for c, j, k in zip(occ, J, K):
+            for d, c in enumerate(occ):
                 Cs.append(c.data_ptr())
                 nocc.append(c.shape[1])
                 ldc.append(max(1, c.stride(0)))
-                Js.append(j.data_ptr())
-                Ks.append(k.data_ptr())
-            outs.append((J, K))
+                Js.append(J[d].data_ptr() if want_j else 0)
+                Ks.append(K[d].data_ptr())
+            outs.append((J, K) if want_j else K)
         n = len(jobs)
         ctx_a = nat.ptr_array(ctxs)
         nd_a = (ctypes.c_int * n)(*nds)
         nocc_a = nat.i64_array(nocc)
         sizes = (ctypes.c_size_t * n)()
-        rc = lib.f2jk_group_workspace_size(ctx_a, n, nd_a, nocc_a, nat.PREC["fp64"],
-                                           nat.F2JK_WANT_J | nat.F2JK_WANT_K, sizes)
+        rc = lib.f2jk_group_workspace_size(ctx_a, n, nd_a, nocc_a, nat.PREC[prec], flags, sizes)
         nat.jk_check(rc, "f2jk_group_workspace_size")
         ws = [jb.fb._workspace(sizes[i]) for i, jb in enumerate(jobs)]
         ws_a = nat.ptr_array([w.data_ptr() for w in ws])
         wsb = (ctypes.c_size_t * n)(*[w.numel() for w in ws])
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
         rc = lib.f2jk_group_build_jk(ctx_a, n, nd_a, nat.ptr_array(Cs), nocc_a, nat.i64_array(ldc),
-                                     nat.ptr_array(Js), nat.ptr_array(Ks), nat.PREC["fp64"],
-                                     nat.F2JK_WANT_J | nat.F2JK_WANT_K, ws_a, wsb, stream)
+                                     nat.ptr_array(Js), nat.ptr_array(Ks), nat.PREC[prec], flags,
+                                     ws_a, wsb, stream)
         nat.jk_check(rc, "f2jk_group_build_jk")
         return outs
 
     def _tick(self, active):
         """Serve the pending build of every active job, then let each take its next step.
 
-        FP64 full builds ("JK") of jobs with an even nbf are batched into
-        grouped launches (jobs with 1 and 2 densities separately); any other
-        request (precision-form builds, odd nbf) runs on its own.
+        "JK" builds (J and K from C) and "K" builds (the factored difference
+        densities of the incremental precision form) are batched into grouped
+        launches by (kind, precision, density count); J builds from a
+        difference density, and jobs the grouped path does not take, run on
+        their own.
         """
         done, results, solo = [], {}, []
         groups = {}
         for jb in active:
             kind, args, prec = jb.pending
-            if self.grouped and kind == "JK" and prec == "fp64" and jb.fb.nbf % 2 == 0:
-                groups.setdefault(len(args), []).append(jb)
+            nd = len(args)
+            if self.grouped and kind in ("JK", "K") and nd <= 2 and self._groupable(jb, kind, prec, nd):
+                groups.setdefault((kind, prec, nd), []).append(jb)
             else:
                 solo.append(jb)
         for jobs in groups.values():
@@ -311,7 +342,7 @@ class SCFServer:
         nbf = _basis.build_basis(req.basis, atoms).size
         naux = _basis.build_basis(req.aux, atoms).size
         nel = sum(z for z, _ in atoms) - req.charge
-        return job_footprint(nbf, naux, nel)
+        return job_footprint(nbf, naux, nel, self.precision)
 
     def summary(self):
         ok = [r for r in self.results.values() if r.error is None and r.converged]
@@ -342,5 +373,25 @@ def serving_stream(nreq=512, rate=4.0, nmax=24, seed=7, basis="cc-pvdz", aux="cc
         n = int(rng.integers(1, nmax + 1))
         xyz = _geom.water_cluster(n, seed=int(rng.integers(0, 2 ** 31)))
         reqs.append(SCFRequest(i, xyz, basis, aux, arrival=t))
+        t += float(rng.exponential(1.0 / rate))
+    return reqs
+
+
+def alkane_stream(nreq=48, rate=0.1, nmin=2, nmax=20, cation_fraction=0.5, seed=11,
+                  basis="cc-pvtz", aux="cc-pvtz-jkfit"):
+    """The mixed-size UHF workload: all-trans C_nH_(2n+2) with n ~ U{nmin..nmax},
+    each either neutral (UHF, ms = 0) or the cation (UHF doublet, ms = 1/2) with
+    probability `cation_fraction`; two densities per J/K build.  Poisson
+    arrivals at `rate` per second."""
+    from . import geometry as _geom
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(nreq):
+        n = int(rng.integers(nmin, nmax + 1))
+        cation = bool(rng.random() < cation_fraction)
+        reqs.append(SCFRequest(i, _geom.alkane(n), basis, aux, charge=1 if cation else 0,
+                               ms=0.5 if cation else 0.0, arrival=t))
         t += float(rng.exponential(1.0 / rate))
     return reqs

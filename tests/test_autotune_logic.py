@@ -21,3 +21,37 @@ def test_save_load_roundtrip(tmp_path):
     p = tmp_path / "t.json"
     autotune.save(t, str(p))
     assert autotune.load(str(p)) == t
+
+
+def test_grouped_entries_are_separate_and_applied():
+    table = {
+        "fp64:576:3336": {"target_blocks": 2048, "xchunk_bytes": 1 << 32, "ms": 9.0},
+        "group:fp64:576:3336": {"target_blocks": 512, "ms": 30.0, "group": 4},
+        "group:bf16x3:576:3336": {"target_blocks": 1024, "bf16_nbw_max": 6, "ms": 12.0, "group": 4},
+    }
+    assert autotune.lookup(600, 3300, "fp64", table)["target_blocks"] == 2048
+    assert autotune.lookup(600, 3300, "fp64", table, grouped=True)["target_blocks"] == 512
+
+    class FB:
+        nbf, naux = 600, 3300
+
+        def __init__(self):
+            self.set = {}
+
+        def set_tuning(self, k, v):
+            self.set[k] = v
+
+    fb = FB()
+    autotune.apply(fb, "bf16x3", table, grouped=True)
+    assert fb.set == {"target_blocks": 1024, "bf16_nbw_max": 6}
+    fb = FB()
+    assert autotune.apply(fb, "fp32", table, grouped=True) is None and fb.set == {}
+
+
+def test_candidate_sets():
+    single64 = list(autotune._candidates("fp64", grouped=False))
+    assert len(single64) == 20 and all("bf16_nbw_max" not in c for c in single64)
+    grouped_bf = list(autotune._candidates("bf16x3", grouped=True))
+    assert len(grouped_bf) == 20 and all("xchunk_bytes" not in c for c in grouped_bf)
+    N, Q, o = autotune.alkane_shapes([20])[0]
+    assert (N, Q, o) == (1188, 2840, [81, 80])

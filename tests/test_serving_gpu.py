@@ -49,6 +49,50 @@ def test_grouped_build_is_bitwise_single_build(dev):
     assert all(torch.equal(a, b) for a, b in zip(outs[0][1], Ks))
 
 
+class _Pending:
+    """A stand-in job with one pending build request."""
+
+    def __init__(self, fb, C, kind="JK", prec="fp64"):
+        self.fb, self.pending = fb, (kind, C, prec)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("kind", ["JK", "K"])
+def test_grouped_precision_forms(dev, prec, kind):
+    # grouped builds of the precision forms: per job bitwise the same alone and
+    # in the mix.  Against the single-job build: J within 1e-6 relative (the
+    # group reads B with another vector width); K within 1e-7 relative of
+    # max|K|.  Most shapes agree bitwise; nbf 200 with occupied widths 64 + 32
+    # (BF16x3) differs by 1.3e-8 in the first density.  The BF16x3 error
+    # itself is ~3e-6.
+    shapes = [(139, 24, [5]), (278, 48, [10]), (417, 72, [15]), (1668, 288, [60]), (700, 200, [33, 32]),
+              (556, 96, [20, 19])]
+    fbs, Cs = [], []
+    for i, (Q, N, noccs) in enumerate(shapes):
+        fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=i, device=dev), device=dev))
+        Cs.append([orthonormal_C(N, o, seed=10 * i + k, device=dev) for k, o in enumerate(noccs)])
+    srv = SCFServer(device=dev)
+    for nd in (1, 2):
+        idx = [i for i, C in enumerate(Cs) if len(C) == nd]
+        mix = srv._build_group([_Pending(fbs[i], Cs[i], kind, prec) for i in idx])
+        for k, i in enumerate(idx):
+            alone = srv._build_group([_Pending(fbs[i], Cs[i], kind, prec)])[0]
+            got = mix[k]
+            if kind == "JK":
+                ref_J, ref_K = fbs[i].build_JK(Cs[i], precision=prec)
+                assert all(torch.equal(a, b) for a, b in zip(alone[0], got[0]))
+                assert all(torch.equal(a, b) for a, b in zip(alone[1], got[1]))
+                for a, b in zip(got[0], ref_J):
+                    assert float((a - b).abs().max()) <= 1e-6 * float(b.abs().max())
+                Kg = got[1]
+            else:
+                ref_K = fbs[i].build_K(Cs[i], precision=prec)
+                assert all(torch.equal(a, b) for a, b in zip(alone, got))
+                Kg = got
+            for a, b in zip(Kg, ref_K):
+                assert float((a - b).abs().max()) <= 1e-7 * float(b.abs().max())
+
+
 def _requests():
     xyzs = [geo.water_cluster(n, seed=100 + n) for n in (1, 2, 3, 5)]
     reqs = [SCFRequest(i, x, arrival=0.0) for i, x in enumerate(xyzs)]
