@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void grouped_kernel(const GArgs a) {
                     const int i = c - g.ioff[d];
                     const double x = (i < g.nocc[d] && k < g.N) ? g.C[d][(long long)k * g.ldc[d] + i] : 0.0;
                     __bf16 h, l;
-                    split_bf16((float)x, h, l);
+                    split_bf16_from_double(x, h, l);
                     g.Cth[t] = __builtin_bit_cast(u16, h);
                     g.Ctl[t] = __builtin_bit_cast(u16, l);
                     if (k < g.N) g.Cp[(long long)k * g.OPAD + c] = x;

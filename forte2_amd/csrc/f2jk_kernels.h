@@ -721,6 +721,15 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
+// A double rounded to float and then split.  The asm keeps the compiler from
+// merging the two roundings (double -> float -> bf16) into one, which can give
+// another hi for a few values: every pad of the coefficients rounds the same way.
+__device__ __forceinline__ void split_bf16_from_double(double x, __bf16& hi, __bf16& lo) {
+    float f = (float)x;
+    asm volatile("" : "+v"(f));
+    split_bf16(f, hi, lo);
+}
+
 // CpT[c][k] (hi/lo) = Cp[k][c] for c < OPAD, k < N; zero for N <= k < KP.
 __global__ void pad_ct_bf16_kernel(const double* __restrict__ C, int64_t ldc, int N, int nocc,
                                    int width, int ioff, int KP, u16* __restrict__ Th,
@@ -728,9 +737,9 @@ __global__ void pad_ct_bf16_kernel(const double* __restrict__ C, int64_t ldc, in
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)width * KP) return;
     const int i = (int)(t / KP), k = (int)(t % KP);
-    const float x = (i < nocc && k < N) ? (float)C[(int64_t)k * ldc + i] : 0.f;
+    const double x = (i < nocc && k < N) ? C[(int64_t)k * ldc + i] : 0.0;
     __bf16 h, l;
-    split_bf16(x, h, l);
+    split_bf16_from_double(x, h, l);
     const int64_t o = (int64_t)(ioff + i) * KP + k;
     Th[o] = __builtin_bit_cast(u16, h);
     Tl[o] = __builtin_bit_cast(u16, l);

@@ -125,6 +125,16 @@ class FockBuilder:
     def set_tuning(self, key, value):
         nat.jk_check(nat.jk_lib().f2jk_set_tuning(self.ctx, key.encode(), int(value)), "f2jk_set_tuning")
 
+    def release(self):
+        """Give back the device memory now: destroy the native context and drop
+        B, its FP32 copy and the workspace (the builder is unusable after).
+        A builder that is part of a reference cycle (System <-> FockBuilder)
+        would otherwise keep B alive until Python's cycle collector runs."""
+        if self._ctx is not None:
+            nat.jk_lib().f2jk_ctx_destroy(self._ctx)
+            self._ctx = None
+        self._B64 = self._B32 = self._ws = None
+
     def __del__(self):
         try:
             if self._ctx is not None:

@@ -126,8 +126,10 @@ class SCFServer:
                  e_tol=1e-8, d_tol=1e-6, maxiter=100, precision="fp64", log=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         if hbm_budget is None:
+            # 85% of the free HBM, less room for the transient of forming one
+            # job's B on the device (the metric factor and the solve blocks)
             free, _ = torch.cuda.mem_get_info(self.device)
-            hbm_budget = int(0.85 * free)
+            hbm_budget = int(0.85 * free) - (4 << 30)
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.prep_threads = int(prep_threads)
@@ -264,6 +266,18 @@ class SCFServer:
         self.ticks += 1
         return done
 
+    def _retire(self, jb):
+        """Release a finished job's device memory at once (its B, workspace and
+        SCF state): System and FockBuilder refer to each other, so dropping the
+        job would leave them to the cycle collector."""
+        if jb.fb is not None:
+            jb.fb.release()
+        if jb.system is not None:
+            jb.system.fock_builder = None
+        jb.gen = jb.scf = jb.system = jb.fb = jb.pending = None
+        jb.__dict__.pop("_keep", None)
+        torch.cuda.empty_cache()
+
     # ------------------------------------------------------------------ main loop
     def run(self, requests):
         """Serve `requests` (arrival times in seconds from now); returns {id: SCFResult}."""
@@ -271,11 +285,12 @@ class SCFServer:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
         queue, preparing, active = [], {}, []
+        ready = []  # (req, fp, prepared) whose activation ran out of HBM: retried as jobs finish
         used = 0
         pool = cf.ThreadPoolExecutor(max_workers=self.prep_threads)
         pi = 0
         try:
-            while pi < len(pending) or queue or preparing or active:
+            while pi < len(pending) or queue or preparing or active or ready:
                 t = now()
                 while pi < len(pending) and pending[pi].arrival <= t:
                     queue.append(pending[pi])
@@ -298,16 +313,32 @@ class SCFServer:
                     self.results[req.id] = res
                     preparing[pool.submit(self._prepare_host, req)] = (req, fp)
                 # jobs whose host work is done join the next tick
+                done_prep = []
                 for fut in [f for f in preparing if f.done()]:
                     req, fp = preparing.pop(fut)
+                    try:
+                        done_prep.append((req, fp, fut.result()))
+                    except Exception as e:  # host preparation failed: this job only
+                        res = self.results[req.id]
+                        res.error, res.finished = repr(e), now()
+                        used -= fp
+                retry, ready = ready, []
+                for req, fp, prepared in retry + done_prep:
                     res = self.results[req.id]
                     try:
-                        job = self._activate(req, fut.result())
+                        job = self._activate(req, prepared)
                         job.footprint = fp
                         job.error = None
                         res.started = now()
                         res.nbf, res.naux = job.fb.nbf, job.fb.naux
                         active.append(job)
+                    except torch.OutOfMemoryError as e:
+                        torch.cuda.empty_cache()
+                        if active:  # memory frees up as jobs finish: try again then
+                            ready.append((req, fp, prepared))
+                        else:
+                            res.error, res.finished = repr(e), now()
+                            used -= fp
                     except Exception as e:  # failed job: others unaffected
                         res.error = repr(e)
                         res.finished = now()
@@ -320,8 +351,7 @@ class SCFServer:
                         res.converged, res.error = bool(jb.scf.converged), jb.error
                         res.finished = now()
                         used -= jb.footprint
-                        jb.fb._ws = None
-                        del jb
+                        self._retire(jb)
                     if self.log:
                         self.log(f"t={now():7.2f}s active={len(active)} preparing={len(preparing)} "
                                  f"queued={len(queue)} done={sum(1 for r in self.results.values() if r.finished)}")
