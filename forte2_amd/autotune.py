@@ -38,7 +38,7 @@ TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning
 CANDIDATES = {
     "target_blocks": [256, 512, 1024, 2048, 4096],
     "xchunk_bytes": [1 << 29, 1 << 30, 1 << 31, 1 << 32],
-    "bf16_nbw_max": [4, 6, 8, 10],
+    "bf16_nbw_max": [4, 5, 6, 8, 10],
 }
 KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max")
 
@@ -128,6 +128,10 @@ def tune_grouped(shapes, precisions=("fp64",), group=4, reps=3, device=None, log
             fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=N + Q + j, device=dev), device=dev))
             Cs.append([orthonormal_C(N, x, seed=N + 7 * j + k, device=dev) for k, x in enumerate(_noccs(o))])
         for prec in precisions:
+            if not srv._groupable(_Pending(fbs[0], Cs[0], prec), "JK", prec, len(Cs[0])):
+                if log:
+                    log(f"[autotune] N={N} Q={Q} o={o} {prec}: not taken by the grouped path (runs alone)")
+                continue
             best = None
             for cand in _candidates(prec, grouped=True):
                 for fb in fbs:

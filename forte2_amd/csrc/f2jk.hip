@@ -60,7 +60,7 @@ struct f2jk_ctx {
     bool j_nontemporal = true;
     int bf16_nbw_max = 10;          // widest BF16x3 half-transform tile: 32 * 10 columns
     // J (HBM-bound) runs on a side stream concurrently with K (MFMA-bound)
-    bool overlap_jk = true;
+    bool overlap_jk = false;
     hipStream_t side = nullptr;     // default priority, created with the context
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int target_blocks = kTargetBlocks;
@@ -203,8 +203,16 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         p.nlower = p.ntile * (p.ntile + 1) / 2;
         p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(ctx->target_blocks / 2, p.nlower)));
         if (precision == F2JK_PREC_BF16X3) {
-            const int need = (int)cdiv(std::min(p.OPAD, 32 * ctx->bf16_nbw_max), 32);
-            p.nbw = need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : 10;
+            // the tile width (in 32-column units) within the cap that computes
+            // the fewest column blocks; ties go to the wider tile
+            const int need = (int)cdiv(p.OPAD, 32);
+            int best = 0, cost = 0;
+            for (int w : {10, 8, 6, 5, 4, 2}) {
+                if (w > ctx->bf16_nbw_max) continue;
+                const int c = (int)cdiv(need, w) * w;
+                if (best == 0 || c < cost) best = w, cost = c;
+            }
+            p.nbw = best;
             p.ncolt = (int)cdiv(p.OPAD, 32 * p.nbw);
         } else {
             p.ncolt = (int)cdiv(p.OPAD, kTile);
@@ -448,6 +456,7 @@ int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipSt
         switch (p.nbw) {
             case 2: launch_kx_bf16<2>(p, grid, A, R, th, tl, xh, xl, st); break;
             case 4: launch_kx_bf16<4>(p, grid, A, R, th, tl, xh, xl, st); break;
+            case 5: launch_kx_bf16<5>(p, grid, A, R, th, tl, xh, xl, st); break;
             case 6: launch_kx_bf16<6>(p, grid, A, R, th, tl, xh, xl, st); break;
             case 8: launch_kx_bf16<8>(p, grid, A, R, th, tl, xh, xl, st); break;
             default: launch_kx_bf16<10>(p, grid, A, R, th, tl, xh, xl, st); break;
@@ -586,6 +595,7 @@ int launch_group_kx_bf16(const GArgs& a, int nbw, hipStream_t st) {
     switch (nbw) {
         case 2: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 2, VA>(a, st);
         case 4: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 4, VA>(a, st);
+        case 5: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 5, VA>(a, st);
         case 6: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 6, VA>(a, st);
         case 8: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 8, VA>(a, st);
         default: return launch_group_phase<GP_KX, F2JK_PREC_BF16X3, 10, VA>(a, st);
@@ -720,8 +730,8 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
     } else if (!strcmp(key, "j_nontemporal")) {
         ctx->j_nontemporal = value != 0;
     } else if (!strcmp(key, "bf16_nbw_max")) {
-        if (value != 2 && value != 4 && value != 6 && value != 8 && value != 10)
-            return fail(F2JK_EINVAL, "bf16_nbw_max must be 2, 4, 6, 8 or 10");
+        if (value != 2 && value != 4 && value != 5 && value != 6 && value != 8 && value != 10)
+            return fail(F2JK_EINVAL, "bf16_nbw_max must be 2, 4, 5, 6, 8 or 10");
         ctx->bf16_nbw_max = (int)value;
     } else if (!strcmp(key, "overlap_jk")) {
         ctx->overlap_jk = value != 0;

@@ -773,11 +773,14 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     nbv = nbv < 0 ? 0 : (nbv > NBW ? NBW : nbv);
 
     constexpr int A_LD = (kTile * kBK16) / (kGThreads * VA);
-    constexpr int B_LD = (BN * kBK16 / 8) / kGThreads;   // 16-byte loads per array
-    static_assert((BN * kBK16 / 8) % kGThreads == 0, "B tile must split evenly");
-    static_assert(2 * B_LD <= NBW, "one B part per odd MFMA group");
+    // the B tile (hi and lo arrays, BN rows of four 16-byte chunks each) is
+    // staged as NBW chunks per thread over both arrays; a wave's 64 chunks lie
+    // in one array
+    constexpr int BCH = BN * (kBK16 / 8);
+    constexpr int B_LD = 2 * BCH / kGThreads;
+    static_assert((2 * BCH) % kGThreads == 0 && BCH % 64 == 0 && B_LD == NBW, "B tile must split evenly");
     Vec<float, VA> ra[A_LD];
-    u32x4 rbh[B_LD], rbl[B_LD];
+    u32x4 rb[B_LD];
 
     // The loads carry no branches: rows past R (or columns past OPAD) are
     // clamped onto the last valid row and feed only output rows/columns that
@@ -796,10 +799,10 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     };
     auto load_b = [&](auto q, int k0) {
         const int idx = tid + kGThreads * q;
-        const int row = idx >> 2, kc = (idx & 3) * 8;
+        const int lo = idx >= BCH, rem = idx - (lo ? BCH : 0);
+        const int row = rem >> 2, kc = (rem & 3) * 8;
         const int64_t o = (int64_t)min(c0 + row, OPAD - 1) * KP + k0 + kc;
-        rbh[q] = *reinterpret_cast<const u32x4*>(Cth + o);
-        rbl[q] = *reinterpret_cast<const u32x4*>(Ctl + o);
+        rb[q] = *reinterpret_cast<const u32x4*>((lo ? Ctl : Cth) + o);
     };
     auto store_a = [&](int stg, auto S0, auto S1) {
         static_for<decltype(S0)::value, decltype(S1)::value>([&](auto S) {
@@ -816,9 +819,9 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     using AE = std::integral_constant<int, A_LD>;
     auto store_b = [&](auto q, int stg) {
         const int idx = tid + kGThreads * q;
-        const int row = idx >> 2, e = swz16(row, idx & 3);
-        *reinterpret_cast<u32x4*>(&Bh[stg][row][e]) = rbh[q];
-        *reinterpret_cast<u32x4*>(&Bl[stg][row][e]) = rbl[q];
+        const int lo = idx >= BCH, rem = idx - (lo ? BCH : 0);
+        const int row = rem >> 2, e = swz16(row, rem & 3);
+        *reinterpret_cast<u32x4*>(lo ? &Bl[stg][row][e] : &Bh[stg][row][e]) = rb[q];
     };
 
     fx4 acc[4][NBW];
@@ -843,8 +846,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 #pragma unroll
         for (int s = 0; s < A_LD; ++s) pin(ra[s]);
 #pragma unroll
-        for (int q = 0; q < B_LD; ++q) {
-        }
+        for (int q = 0; q < B_LD; ++q) pin(rb[q]);
     }
     const int fr = lane & 15, fc = lane >> 4;  // fragment row / 16-byte chunk of this lane
     for (int kt = 0; kt < nk; ++kt) {
@@ -892,10 +894,9 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
                 store_a(nxt, AH{}, AE{});
                 load_a(kpre, AH{}, AE{});
             }
-            if constexpr ((j & 1) && (j >> 1) < B_LD) {
-                store_b(std::integral_constant<int, (j >> 1)>{}, nxt);
-                load_b(std::integral_constant<int, (j >> 1)>{}, kpre);
-            }
+            // one B chunk per group
+            store_b(std::integral_constant<int, j>{}, nxt);
+            load_b(std::integral_constant<int, j>{}, kpre);
             __builtin_amdgcn_sched_barrier(0);
         });
         __syncthreads();
@@ -918,8 +919,13 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
             }
         }
 }
+// Tiles of at most 5 column blocks need at most 256 registers: two waves per
+// SIMD, i.e. two independent workgroups per CU that hide each other's stalls.
+template <int NBW>
+constexpr int kx_bf16_waves_per_eu() { return NBW <= 5 ? 2 : 1; }
+
 template <int NBW, int VA>
-__global__ __launch_bounds__(kGThreads) void kx_bf16x3_kernel(
+__global__ __launch_bounds__(kGThreads, kx_bf16_waves_per_eu<NBW>()) void kx_bf16x3_kernel(
     const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
     const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt) {
     kx_bf16x3_kernel_body<NBW, VA>(A, R, K, KP, Cth, Ctl, OPAD, Xh, Xl, ncolt, blockIdx.x, gridDim.x);

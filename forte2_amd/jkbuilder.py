@@ -58,6 +58,8 @@ class FockBuilder:
         self._B32 = None
         self._ctx = None
         self._ws = None
+        self._manual = set()  # launch-shape knobs set by hand
+        self._tuned = None    # precision whose tuned launch shape is applied
         self.timing = False
         self.last_timings = None
         self.last_launches = None
@@ -123,7 +125,27 @@ class FockBuilder:
         )
 
     def set_tuning(self, key, value):
+        """Set a launch-shape knob (f2jk_set_tuning).  A knob set here is kept:
+        the autotuned table no longer changes it."""
+        self._manual.add(key)
+        self._tune(key, value)
+
+    def _tune(self, key, value):
         nat.jk_check(nat.jk_lib().f2jk_set_tuning(self.ctx, key.encode(), int(value)), "f2jk_set_tuning")
+
+    def _apply_table(self, prec):
+        """Launch shape of this size class and precision from the autotuned
+        table (forte2_amd/data/tuning_gfx950.json), for knobs not set by hand."""
+        if self._tuned == prec:
+            return
+        self._tuned = prec
+        from . import autotune
+
+        e = autotune.lookup(self.nbf, self.naux, prec)
+        if e is not None:
+            for k in autotune.KEYS:
+                if k in e and k not in self._manual:
+                    self._tune(k, e[k])
 
     def release(self):
         """Give back the device memory now: destroy the native context and drop
@@ -153,6 +175,7 @@ class FockBuilder:
             raise ValueError(f"unknown precision {p!r}")
         if p != "fp64":
             self.B_fp32()
+        self._apply_table(p)
         return nat.PREC[p]
 
     # ------------------------------------------------------------------ helpers

@@ -90,10 +90,11 @@ int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
  * workspace (default 4 GiB); "target_blocks" is the number of workgroups a
  * split launch aims for (default 2048); "j_nontemporal" (0/1, default 1)
  * streams B through non-temporal loads in the J passes; "overlap_jk" (0/1,
- * default 1) runs the J passes of a build_jk on the context's side stream
+ * default 0) runs the J passes of a build_jk on the context's side stream
  * concurrently with K (joined back before the call returns its work to the
- * caller's stream); "bf16_nbw_max" (2, 4, 6, 8, 10) caps the BF16x3
- * half-transform tile at 32 * value occupied columns.  Set by the autotuner. */
+ * caller's stream); "bf16_nbw_max" (2, 4, 5, 6, 8, 10) caps the BF16x3
+ * half-transform tile at 32 * value occupied columns (tiles of up to 5 run two
+ * workgroups per CU).  Set by the autotuner. */
 int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value);
 
 /* Scratch bytes needed by f2jk_build_jk / _j / _k for these arguments.

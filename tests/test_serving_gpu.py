@@ -24,6 +24,7 @@ def test_grouped_build_is_bitwise_single_build(dev):
     fbs, Cs = [], []
     for i, (Q, N, noccs) in enumerate(shapes):
         fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=i, device=dev), device=dev))
+        fbs[-1].set_tuning("target_blocks", 2048)  # one plan for grouped and single builds
         Cs.append([orthonormal_C(N, o, seed=10 * i + k, device=dev) for k, o in enumerate(noccs)])
     singles = [fb.build_JK(C) for fb, C in zip(fbs, Cs)]
     srv = SCFServer(device=dev)
@@ -70,6 +71,7 @@ def test_grouped_precision_forms(dev, prec, kind):
     fbs, Cs = [], []
     for i, (Q, N, noccs) in enumerate(shapes):
         fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=i, device=dev), device=dev))
+        fbs[-1].set_tuning("target_blocks", 2048)  # one plan for grouped and single builds
         Cs.append([orthonormal_C(N, o, seed=10 * i + k, device=dev) for k, o in enumerate(noccs)])
     srv = SCFServer(device=dev)
     for nd in (1, 2):
