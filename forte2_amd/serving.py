@@ -156,8 +156,12 @@ class SCFServer:
         system.device = self.device
         B = _df.device_B_from_host(M, T, self.device)
         fb = FockBuilder(system, B=B, device=self.device)
-        # launch shape of the job's size class (tuned for grouped builds when batching)
+        # launch shape of the job's size class (tuned for grouped builds when
+        # batching); the X chunk is pinned so that the job's builds that run
+        # alone keep the plan the grouped path was asked about
         autotune.apply(fb, self.precision, grouped=self.grouped)
+        if self.grouped:
+            fb.set_tuning("xchunk_bytes", 1 << 32)
         system.fock_builder = fb
         kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter, precision=self.precision)
         scf = RHF(req.charge, **kw) if req.ms is None else UHF(req.charge, ms=req.ms, **kw)
@@ -247,10 +251,17 @@ class SCFServer:
                 groups.setdefault((kind, prec, nd), []).append(jb)
             else:
                 solo.append(jb)
-        for jobs in groups.values():
+        for (kind, prec, nd), jobs in groups.items():
             for g0 in range(0, len(jobs), MAX_GROUP):
                 grp = jobs[g0:g0 + MAX_GROUP]
-                for jb, out in zip(grp, self._build_group(grp)):
+                try:
+                    outs = self._build_group(grp)
+                except ValueError:  # the library declines the group: these jobs run alone from now on
+                    for jb in grp:
+                        jb._groupable[(kind, prec, nd)] = False
+                    solo.extend(grp)
+                    continue
+                for jb, out in zip(grp, outs):
                     results[id(jb)] = out
                 self.group_sizes.append(len(grp))
         for jb in solo:
