@@ -36,7 +36,7 @@ import time
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning_gfx950.json")
 
 CANDIDATES = {
-    "target_blocks": [256, 512, 1024, 2048, 4096],
+    "target_blocks": [256, 512, 1024, 2048, 4096, 8192, 16384],
     "xchunk_bytes": [1 << 29, 1 << 30, 1 << 31, 1 << 32],
     "bf16_nbw_max": [4, 5, 6, 8, 10],
 }

@@ -735,6 +735,7 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
         ctx->bf16_nbw_max = (int)value;
     } else if (!strcmp(key, "overlap_jk")) {
         ctx->overlap_jk = value != 0;
+
     } else if (!strcmp(key, "target_blocks")) {
         if (value < 1 || value > (1 << 20)) return fail(F2JK_EINVAL, "target_blocks out of range");
         ctx->target_blocks = (int)value;

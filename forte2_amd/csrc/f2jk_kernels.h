@@ -934,7 +934,9 @@ __global__ __launch_bounds__(kGThreads, kx_bf16_waves_per_eu<NBW>()) void kx_bf1
 
 // KS (bf16x3): Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
 // with X given as hi/lo bf16 pairs; width % 32 == 0.  FP32 accumulation over
-// one P, folded into FP64 registers after every P.
+// the P range of a split (the MFMA rounds once per 32 products), FP64 across
+// splits.  With no FP64 registers the kernel fits two workgroups per CU; the
+// split length (target_blocks) sets the length of the FP32 sums.
 __device__ __forceinline__ void ks_bf16x3_kernel_body(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
     int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate, int b, int nblocks) {
@@ -1002,15 +1004,11 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         }
     };
 
-    dx4 acc64[4][4];
     fx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc64[i][j] = dx4{0, 0, 0, 0};
-            acc[i][j] = fx4{0, 0, 0, 0};
-        }
+        for (int j = 0; j < 4; ++j) acc[i][j] = fx4{0, 0, 0, 0};
 
     const int nkp = width / kBK16;
     // steps are numbered across P; step st+1's tiles are written to the other
@@ -1080,15 +1078,6 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         });
         __syncthreads();
     }
-    // end of one P: fold the FP32 partials into FP64
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc64[i][j][r] += (double)acc[i][j][r];
-            acc[i][j] = fx4{0, 0, 0, 0};
-        }
     }
     double* Ks = Kpart + (int64_t)s * N * N;
 #pragma unroll
@@ -1102,14 +1091,16 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
                 const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
                 if (row >= N) continue;
                 double* dst = Ks + (int64_t)row * N + col;
-                *dst = accumulate ? (*dst + acc64[i][j][r]) : acc64[i][j][r];
+                const double v = (double)acc[i][j][r];
+                *dst = accumulate ? (*dst + v) : v;
             }
         }
 }
-__global__ __launch_bounds__(kGThreads) void ks_bf16x3_kernel(
+__global__ __launch_bounds__(kGThreads, 2) void ks_bf16x3_kernel(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
     int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate) {
-    ks_bf16x3_kernel_body(Xh, Xl, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate, blockIdx.x, gridDim.x);
+    ks_bf16x3_kernel_body(Xh, Xl, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate, blockIdx.x,
+                          gridDim.x);
 }
 
 

@@ -52,7 +52,8 @@ enum f2jk_dtype { F2JK_F64 = 0, F2JK_F32 = 1 };
  * FP32: B32 for J (FP64 accumulation) and FP32 MFMA for K (FP32 accumulation
  * within a P block, FP64 across blocks).
  * BF16X3: B32 for J; K operands split into bf16 hi/lo pairs, three bf16 MFMA
- * products (hi*hi + hi*lo + lo*hi) with FP32 accumulation. */
+ * products (hi*hi + hi*lo + lo*hi) with FP32 accumulation (over the P range of
+ * one split in the contraction, FP64 across splits). */
 enum f2jk_precision { F2JK_PREC_FP64 = 0, F2JK_PREC_FP32 = 1, F2JK_PREC_BF16X3 = 2 };
 
 /* what f2jk_build_jk computes; F2JK_TIMING records per-phase HIP events */
