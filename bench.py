@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "JK builds/s + MFMA util, (H2O)_64 mixed vs FP64 at 1e-8 Eh; p99 SCF latency"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+# HBM bytes per launch of the half-transform at the default workload, measured
+# with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size:
+# x2 and x1) on these kernels and launch shapes (profiles/pmc_traffic_h2o64_bf16x3.md);
+# other workloads report null.
+MEASURED_KX_TRAFFIC = {(64, "bf16x3"): 19.50e9 + 3.55e9}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -291,6 +296,8 @@ def main():
     kx_flops = 2.0 * Q * N * N * o          # algorithmic half-transform flops of one build
     kx_ms_launch = ph_mix["kx"] / max(1, la_mix["kx"])
     kx_flops_launch = kx_flops / max(1, la_mix["kx"])
+    # B read once plus X written once, per launch (FP32 B, hi/lo bf16 X)
+    kx_bytes_launch = (4.0 * Q * N * N + 4.0 * Q * N * 32 * ((o + 31) // 32)) / max(1, la_mix["kx"])
     achieved = kx_flops_launch / (kx_ms_launch * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.mixed]
     k_ms = ph_mix["kx"] + ph_mix["ks"] + ph_mix["k_reduce"]
@@ -339,13 +346,16 @@ def main():
             "flops_algorithmic": kf, "flops_full_gemm": 4.0 * Q * N * N * o,
         },
         "roofline": {
-            "kernel": "kx_kernel (K half-transform X = B C)",
+            "kernel": "kx_bf16x3_kernel (K half-transform X = B C)" if args.mixed == "bf16x3"
+                      else "kx_kernel (K half-transform X = B C)",
             "bound": "mfma",
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": MEASURED_KX_TRAFFIC.get((args.waters, args.mixed)),
+            "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (copy-calibrated); "
+                            "algorithmic bytes per launch %.3g" % kx_bytes_launch,
             "launches_per_build": la_mix["kx"],
             "avg_launch_ms": kx_ms_launch,
         },
