@@ -11,7 +11,8 @@ the launch shape:
     trades L2/MALL reuse between KX and KS against launch count;
   * ``bf16_nbw_max`` (BF16x3): the widest half-transform tile, 32*NBW
     columns.  Narrower tiles use fewer registers and LDS but re-read the B
-    rows once per column tile.
+    rows once per column tile;
+  * ``kx_waves`` (BF16x3): 4 or 8 waves per half-transform workgroup.
 
 ``tune(shapes)`` times every candidate on synthetic B of each shape with HIP
 events and keeps the fastest; ``tune_grouped(shapes)`` does the same for a
@@ -39,8 +40,11 @@ CANDIDATES = {
     "target_blocks": [256, 512, 1024, 2048, 4096, 8192, 16384],
     "xchunk_bytes": [1 << 29, 1 << 30, 1 << 31, 1 << 32],
     "bf16_nbw_max": [4, 5, 6, 8, 10],
+    "kx_waves": [4, 8],
 }
-KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max")
+# the BF16x3 grid is larger (tile cap, waves): fewer chunk sizes there
+XCHUNK_BF16 = [1 << 31, 1 << 32]
+KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max", "kx_waves")
 
 
 def _key(nbf, naux, precision, grouped=False):
@@ -55,7 +59,10 @@ def _candidates(precision, grouped):
     names = ["target_blocks"] + ([] if grouped else ["xchunk_bytes"])
     if precision == "bf16x3":
         names.append("bf16_nbw_max")
-    for vals in itertools.product(*[CANDIDATES[n] for n in names]):
+        if not grouped:  # grouped half-transforms use 4-wave workgroups
+            names.append("kx_waves")
+    pools = {**CANDIDATES, "xchunk_bytes": XCHUNK_BF16 if precision == "bf16x3" else CANDIDATES["xchunk_bytes"]}
+    for vals in itertools.product(*[pools[n] for n in names]):
         yield dict(zip(names, vals))
 
 

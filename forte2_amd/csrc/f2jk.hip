@@ -61,6 +61,7 @@ struct f2jk_ctx {
     int bf16_nbw_max = 10;          // widest BF16x3 half-transform tile: 32 * 10 columns
     // J (HBM-bound) runs on a side stream concurrently with K (MFMA-bound)
     bool overlap_jk = false;
+    int kx_waves = 8;               // waves per BF16x3 half-transform workgroup (4 or 8)
     hipStream_t side = nullptr;     // default priority, created with the context
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int target_blocks = kTargetBlocks;
@@ -90,7 +91,7 @@ struct Plan {
     int jvec = 1, nvec = 0, neg = 0, npc1 = 0, ppw2 = 0, npc2 = 0;
     size_t jb = 8;             // bytes of a B element read by J
     // K
-    int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0;
+    int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0, kx_waves = 4;
     // workspace offsets
     int KP = 0;                // nbf rounded up to the bf16 k step
     size_t off_cp32 = 0, off_cth = 0, off_ctl = 0;
@@ -213,6 +214,7 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
                 if (best == 0 || c < cost) best = w, cost = c;
             }
             p.nbw = best;
+            p.kx_waves = ctx->kx_waves;
             p.ncolt = (int)cdiv(p.OPAD, 32 * p.nbw);
         } else {
             p.ncolt = (int)cdiv(p.OPAD, kTile);
@@ -423,18 +425,29 @@ int run_k_typed(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStr
     return F2JK_OK;
 }
 
+template <int NBW, int WAVES>
+void launch_kx_bf16_w(const Plan& p, unsigned grid, const float* A, int64_t R, const u16* th,
+                      const u16* tl, u16* xh, u16* xl, hipStream_t st) {
+    if (p.kva == 4)
+        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 4, WAVES>), dim3(grid), dim3(64 * WAVES), 0, st, A, R, p.N,
+                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+    else if (p.kva == 2)
+        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 2, WAVES>), dim3(grid), dim3(64 * WAVES), 0, st, A, R, p.N,
+                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+    else
+        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 1, WAVES>), dim3(grid), dim3(64 * WAVES), 0, st, A, R, p.N,
+                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+}
+
+// 8 waves for tiles of an even number of 32-column units (each wave then
+// holds NBW/2 blocks of 16 columns); 4 waves otherwise
 template <int NBW>
 void launch_kx_bf16(const Plan& p, unsigned grid, const float* A, int64_t R, const u16* th,
                     const u16* tl, u16* xh, u16* xl, hipStream_t st) {
-    if (p.kva == 4)
-        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 4>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N,
-                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
-    else if (p.kva == 2)
-        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 2>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N,
-                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
-    else
-        hipLaunchKernelGGL((kx_bf16x3_kernel<NBW, 1>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N,
-                           p.KP, th, tl, p.OPAD, xh, xl, p.ncolt);
+    if constexpr (NBW % 2 == 0 && NBW >= 4) {
+        if (p.kx_waves == 8) return launch_kx_bf16_w<NBW, 8>(p, grid, A, R, th, tl, xh, xl, st);
+    }
+    launch_kx_bf16_w<NBW, 4>(p, grid, A, R, th, tl, xh, xl, st);
 }
 
 int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStream_t st) {
@@ -735,6 +748,9 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
         ctx->bf16_nbw_max = (int)value;
     } else if (!strcmp(key, "overlap_jk")) {
         ctx->overlap_jk = value != 0;
+    } else if (!strcmp(key, "kx_waves")) {
+        if (value != 4 && value != 8) return fail(F2JK_EINVAL, "kx_waves must be 4 or 8");
+        ctx->kx_waves = (int)value;
 
     } else if (!strcmp(key, "target_blocks")) {
         if (value < 1 || value > (1 << 20)) return fail(F2JK_EINVAL, "target_blocks out of range");

@@ -750,12 +750,18 @@ __global__ void pad_ct_bf16_kernel(const double* __restrict__ C, int64_t ldc, in
 //   Ct = pre-split transposed coefficients, OPAD x KP (KP = K rounded up to 32)
 //   X  = hi/lo bf16 pairs, R x OPAD.
 // Tile 128 x (32*NBW): one block holds up to 320 occupied columns so that B is
-// read from HBM once.  4 waves as 2 x 2, each 64 x (16*NBW).
-template <int NBW, int VA>
+// read from HBM once.  WAVES waves as 2 x (WAVES/2), each 64 rows x NB blocks
+// of 16 columns (NB = 4*NBW/WAVES).  With 8 waves (two per SIMD, ~190
+// registers each) the tile's staging is shared by twice the waves.
+template <int NBW, int VA, int WAVES = 4>
 __device__ __forceinline__ void kx_bf16x3_kernel_body(
     const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
     const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt, int b, int nblocks) {
     constexpr int BN = 32 * NBW;
+    constexpr int T = 64 * WAVES;       // threads
+    constexpr int WC = WAVES / 2;       // waves across the columns
+    constexpr int NB = 2 * NBW / WC;    // 16-column blocks per wave
+    static_assert((2 * NBW) % WC == 0 && NB >= 1, "tile must split over the waves");
     // two LDS stages: the MFMAs read one while the next k-step is written to the other
     __shared__ __bf16 Ah[2][kTile][kBK16];
     __shared__ __bf16 Al[2][kTile][kBK16];
@@ -767,18 +773,18 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     const int t = xcd_remap(b, nblocks);
     const int64_t r0 = (int64_t)(t / ncolt) * kTile;
     const int c0 = (t % ncolt) * BN;
-    const int wr = wid >> 1, wc = wid & 1;
-    const int wcol0 = c0 + wc * 16 * NBW;
+    const int wr = wid / WC, wc = wid % WC;
+    const int wcol0 = c0 + wc * 16 * NB;
     int nbv = (OPAD - wcol0) / 16;
-    nbv = nbv < 0 ? 0 : (nbv > NBW ? NBW : nbv);
+    nbv = nbv < 0 ? 0 : (nbv > NB ? NB : nbv);
 
-    constexpr int A_LD = (kTile * kBK16) / (kGThreads * VA);
+    constexpr int A_LD = (kTile * kBK16) / (T * VA);
     // the B tile (hi and lo arrays, BN rows of four 16-byte chunks each) is
-    // staged as NBW chunks per thread over both arrays; a wave's 64 chunks lie
-    // in one array
+    // staged as NB chunks per thread over both arrays (one per MFMA group); a
+    // wave's 64 chunks lie in one array
     constexpr int BCH = BN * (kBK16 / 8);
-    constexpr int B_LD = 2 * BCH / kGThreads;
-    static_assert((2 * BCH) % kGThreads == 0 && BCH % 64 == 0 && B_LD == NBW, "B tile must split evenly");
+    constexpr int B_LD = 2 * BCH / T;
+    static_assert((2 * BCH) % T == 0 && BCH % 64 == 0 && B_LD == NB, "B tile must split evenly");
     Vec<float, VA> ra[A_LD];
     u32x4 rb[B_LD];
 
@@ -790,7 +796,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     auto load_a = [&](int k0, auto S0, auto S1) {
         static_for<decltype(S0)::value, decltype(S1)::value>([&](auto S) {
             constexpr int s = decltype(S)::value;
-            const int idx = tid + kGThreads * s;
+            const int idx = tid + T * s;
             const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
             const int64_t gr = min(r0 + row, R - 1);
             const int gk = min(k0 + kc, K - VA);
@@ -798,7 +804,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
         });
     };
     auto load_b = [&](auto q, int k0) {
-        const int idx = tid + kGThreads * q;
+        const int idx = tid + T * q;
         const int lo = idx >= BCH, rem = idx - (lo ? BCH : 0);
         const int row = rem >> 2, kc = (rem & 3) * 8;
         const int64_t o = (int64_t)min(c0 + row, OPAD - 1) * KP + k0 + kc;
@@ -807,7 +813,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
     auto store_a = [&](int stg, auto S0, auto S1) {
         static_for<decltype(S0)::value, decltype(S1)::value>([&](auto S) {
             constexpr int s = decltype(S)::value;
-            const int idx = tid + kGThreads * s;
+            const int idx = tid + T * s;
             const int row = idx / (kBK16 / VA), kc = (idx % (kBK16 / VA)) * VA;
             const int e = swz16(row, kc >> 3) + (kc & 7);
 #pragma unroll
@@ -815,20 +821,20 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
         });
     };
     using A0 = std::integral_constant<int, 0>;
-    using AH = std::integral_constant<int, (NBW > 2 ? A_LD / 2 : A_LD)>;
+    using AH = std::integral_constant<int, (NB > 2 ? A_LD / 2 : A_LD)>;
     using AE = std::integral_constant<int, A_LD>;
     auto store_b = [&](auto q, int stg) {
-        const int idx = tid + kGThreads * q;
+        const int idx = tid + T * q;
         const int lo = idx >= BCH, rem = idx - (lo ? BCH : 0);
         const int row = rem >> 2, e = swz16(row, rem & 3);
         *reinterpret_cast<u32x4*>(lo ? &Bl[stg][row][e] : &Bh[stg][row][e]) = rb[q];
     };
 
-    fx4 acc[4][NBW];
+    fx4 acc[4][NB];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < NBW; ++j) acc[i][j] = fx4{0, 0, 0, 0};
+        for (int j = 0; j < NB; ++j) acc[i][j] = fx4{0, 0, 0, 0};
 
     const int nk = KP / kBK16;
     load_a(0, A0{}, AE{});
@@ -857,7 +863,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
         // fragment reads in the order the first group consumes them (bh, al for
         // the first sweep, then bl, ah), so its first MFMA waits for two reads
         bf16x8 ah[4], al[4], bh[2], bl[2];
-        const int rb0 = wc * 16 * NBW + fr;
+        const int rb0 = wc * 16 * NB + fr;
         bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][rb0][swz16(rb0, fc)]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -875,10 +881,10 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
         // never depend on each other), and the staging work of the next steps
         // sits in fixed groups so that it issues under the MFMAs.  Every block
         // is computed, also those past OPAD (their B rows are clamped copies).
-        static_for<0, NBW>([&](auto J) {
+        static_for<0, NB>([&](auto J) {
             constexpr int j = decltype(J)::value;
-            if constexpr (j + 1 < NBW) {
-                const int r = wc * 16 * NBW + (j + 1) * 16 + fr;
+            if constexpr (j + 1 < NB) {
+                const int r = wc * 16 * NB + (j + 1) * 16 + fr;
                 bh[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
                 bl[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
             }
@@ -904,7 +910,7 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < NBW; ++j) {
+        for (int j = 0; j < NB; ++j) {
             if (j >= nbv) continue;
             const int col = wcol0 + j * 16 + (lane & 15);
 #pragma unroll
@@ -919,16 +925,16 @@ __device__ __forceinline__ void kx_bf16x3_kernel_body(
             }
         }
 }
-// Tiles of at most 5 column blocks need at most 256 registers: two waves per
-// SIMD, i.e. two independent workgroups per CU that hide each other's stalls.
-template <int NBW>
-constexpr int kx_bf16_waves_per_eu() { return NBW <= 5 ? 2 : 1; }
+// Waves of at most 5 column blocks need at most 256 registers: two waves per
+// SIMD (two 4-wave workgroups per CU, or one 8-wave workgroup).
+template <int NBW, int WAVES = 4>
+constexpr int kx_bf16_waves_per_eu() { return (WAVES == 8 || NBW <= 5) ? 2 : 1; }
 
-template <int NBW, int VA>
-__global__ __launch_bounds__(kGThreads, kx_bf16_waves_per_eu<NBW>()) void kx_bf16x3_kernel(
+template <int NBW, int VA, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, (kx_bf16_waves_per_eu<NBW, WAVES>())) void kx_bf16x3_kernel(
     const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
     const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt) {
-    kx_bf16x3_kernel_body<NBW, VA>(A, R, K, KP, Cth, Ctl, OPAD, Xh, Xl, ncolt, blockIdx.x, gridDim.x);
+    kx_bf16x3_kernel_body<NBW, VA, WAVES>(A, R, K, KP, Cth, Ctl, OPAD, Xh, Xl, ncolt, blockIdx.x, gridDim.x);
 }
 
 

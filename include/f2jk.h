@@ -95,7 +95,9 @@ int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
  * concurrently with K (joined back before the call returns its work to the
  * caller's stream); "bf16_nbw_max" (2, 4, 5, 6, 8, 10) caps the BF16x3
  * half-transform tile at 32 * value occupied columns (tiles of up to 5 run two
- * workgroups per CU).  Set by the autotuner. */
+ * workgroups per CU); "kx_waves" (4 or 8, default 8) is the number of waves of
+ * a BF16x3 half-transform workgroup (8: two per SIMD sharing the tile; tiles
+ * of an even number of 32-column units).  Set by the autotuner. */
 int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value);
 
 /* Scratch bytes needed by f2jk_build_jk / _j / _k for these arguments.

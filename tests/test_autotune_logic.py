@@ -52,6 +52,8 @@ def test_candidate_sets():
     single64 = list(autotune._candidates("fp64", grouped=False))
     assert len(single64) == 28 and all("bf16_nbw_max" not in c for c in single64)
     grouped_bf = list(autotune._candidates("bf16x3", grouped=True))
-    assert len(grouped_bf) == 35 and all("xchunk_bytes" not in c for c in grouped_bf)
+    assert len(grouped_bf) == 35 and all("xchunk_bytes" not in c and "kx_waves" not in c for c in grouped_bf)
+    single_bf = list(autotune._candidates("bf16x3", grouped=False))
+    assert len(single_bf) == 140 and {c["kx_waves"] for c in single_bf} == {4, 8}
     N, Q, o = autotune.alkane_shapes([20])[0]
     assert (N, Q, o) == (1188, 2840, [81, 80])
