@@ -34,6 +34,7 @@ F2JK_SYMBOLS = (
     "f2jk_work",
     "f2jk_group_workspace_size",
     "f2jk_group_build_jk",
+    "f2jk_half_transform",
     "f2jk_last_error",
 )
 
@@ -79,6 +80,7 @@ def jk_lib():
     lib.f2jk_group_workspace_size.argtypes = [c_voidp_p, I, ctypes.POINTER(I), c_int64_p, I, I, ctypes.POINTER(sz)]
     lib.f2jk_group_build_jk.argtypes = [c_voidp_p, I, ctypes.POINTER(I), c_voidp_p, c_int64_p, c_int64_p, c_voidp_p,
                                         c_voidp_p, I, I, c_voidp_p, ctypes.POINTER(sz), V]
+    lib.f2jk_half_transform.argtypes = [V, V, I64, I64, I64, I64, V, I64, I, V, sz, V]
     lib.f2jk_last_error.argtypes = []
     lib.f2jk_last_error.restype = ctypes.c_char_p
     for s in F2JK_SYMBOLS:

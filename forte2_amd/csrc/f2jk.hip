@@ -952,6 +952,46 @@ int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens, cons
     return F2JK_OK;
 }
 
+int f2jk_half_transform(f2jk_ctx* ctx, const double* C, int64_t ncol, int64_t ldc, int64_t p0, int64_t p1,
+                        double* X, int64_t ldx, int precision, void* ws, size_t ws_bytes, void* stream) {
+    if (!ctx) return fail(F2JK_EINVAL, "null context");
+    if (!C || !X) return fail(F2JK_EINVAL, "C and X are required");
+    if (precision != F2JK_PREC_FP64) return fail(F2JK_EUNSUPPORTED, "the half-transform is FP64");
+    if (ncol < 1 || ldc < ncol || ldx < ncol) return fail(F2JK_EINVAL, "ncol, ldc or ldx invalid");
+    Plan p;
+    int rc = make_plan(ctx, 1, &ncol, precision, F2JK_WANT_K, false, p);
+    if (rc) return rc;
+    if (p0 < 0 || p1 > p.Q || p0 >= p1) return fail(F2JK_EINVAL, "aux range [p0, p1) invalid");
+    if ((rc = check_ws(p, ws, ws_bytes))) return rc;
+    F2JK_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    char* w = (char*)ws;
+    const double* Cs[1] = {C};
+    const int64_t lds[1] = {ldc};
+    if ((rc = pack_c(ctx, p, Cs, &ncol, lds, w, st, false))) return rc;
+    const double* B = (const double*)ctx->B[F2JK_F64];
+    const double* cp = (const double*)(w + p.off_cp);
+    double* Xc = (double*)(w + p.off_x);
+    const int64_t NN = (int64_t)p.N * p.N;
+    for (int64_t P0 = p0; P0 < p1; P0 += p.Pc) {
+        const int pc = (int)std::min<int64_t>(p.Pc, p1 - P0);
+        const int64_t R = (int64_t)pc * p.N;
+        const unsigned grid = (unsigned)(cdiv(R, kTile) * p.ncolt);
+        if (p.kva == 2)
+            hipLaunchKernelGGL((kx_kernel<double, 2>), dim3(grid), dim3(kGThreads), 0, st, B + (int64_t)P0 * NN, R,
+                               p.N, cp, p.OPAD, Xc, p.ncolt);
+        else
+            hipLaunchKernelGGL((kx_kernel<double, 1>), dim3(grid), dim3(kGThreads), 0, st, B + (int64_t)P0 * NN, R,
+                               p.N, cp, p.OPAD, Xc, p.ncolt);
+        F2JK_LAUNCHED("kx");
+        hipLaunchKernelGGL(x_copy_kernel, dim3((unsigned)cdiv(R * ncol, 256)), dim3(256), 0, st, Xc, R, p.OPAD,
+                           (int)ncol, X + (P0 - p0) * p.N * ldx, ldx);
+        F2JK_LAUNCHED("x_copy");
+    }
+    g_last_error.clear();
+    return F2JK_OK;
+}
+
 const char* f2jk_last_error(void) { return g_last_error.c_str(); }
 
 }  // extern "C"

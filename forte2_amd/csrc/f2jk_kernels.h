@@ -1110,6 +1110,17 @@ __global__ __launch_bounds__(kGThreads, 2) void ks_bf16x3_kernel(
 }
 
 
+// out[r][c] = X[r][c] for r < R, c < ncol: a half-transformed chunk (row
+// stride OPAD) copied to a caller buffer with row stride ldx.
+__global__ void x_copy_kernel(const double* __restrict__ X, int64_t R, int OPAD, int ncol,
+                              double* __restrict__ out, int64_t ldx) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= R * ncol) return;
+    const int64_t r = t / ncol;
+    const int c = (int)(t % ncol);
+    out[r * ldx + c] = X[r * OPAD + c];
+}
+
 // K[m][n] = K[n][m] = sum_s Kpart[s][m][n] for m >= n (fixed order in s).
 __global__ void k_reduce_kernel(const double* __restrict__ Kpart, int nsplit, int N,
                                 double* __restrict__ K) {

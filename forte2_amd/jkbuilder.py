@@ -325,6 +325,66 @@ class FockBuilder:
         """F = F_core + F_act (jkbuilder.py:351-380)."""
         return self.build_core_fock(C_core, hcore, precision) + self.build_active_fock(C_act, g1, precision)
 
+    # ------------------------------------------------------------------ MO-basis blocks
+    def _x_blocks(self, C, block_bytes=1 << 30):
+        """Yields (P0, P1, X) with X[P-P0, m, i] = sum_n B_Pmn C_ni (FP64, on the
+        device, f2jk_half_transform) for blocks of P of at most block_bytes."""
+        lib = nat.jk_lib()
+        t, ld = self._rowmajor(C)
+        N, Q, ncol = self.nbf, self.naux, int(t.shape[1])
+        nbytes = ctypes.c_size_t()
+        nocc_a = nat.i64_array([ncol])
+        nat.jk_check(lib.f2jk_workspace_size(self.ctx, 1, nocc_a, nat.PREC["fp64"], nat.F2JK_WANT_K,
+                                             ctypes.byref(nbytes)), "workspace")
+        ws = self._workspace(nbytes.value)
+        step = max(1, min(Q, block_bytes // (8 * N * ncol)))
+        for P0 in range(0, Q, step):
+            P1 = min(Q, P0 + step)
+            X = torch.empty((P1 - P0, N, ncol), dtype=torch.float64, device=self.device)
+            nat.jk_check(lib.f2jk_half_transform(self.ctx, ctypes.c_void_p(t.data_ptr()), ncol, ld, P0, P1,
+                                                 ctypes.c_void_p(X.data_ptr()), ncol, nat.PREC["fp64"],
+                                                 ctypes.c_void_p(ws.data_ptr()), ws.numel(), self._stream()),
+                         "f2jk_half_transform")
+            yield P0, P1, X
+
+    def B_tensor_gen_block(self, C1, C2):
+        """B[P, i, j] = sum_mn B_Pmn C1_mi C2_nj: the MO-basis B tensor
+        (jkbuilder.py:521-552).  X = B C2 on MFMA, then C1^T X as a batched
+        GEMM, by blocks of P.  Real orbitals (conj is the identity)."""
+        self.B_Pmn
+        c1, to_np = self._as_dev(C1)
+        c2, to_np2 = self._as_dev(C2)
+        if c1.dim() != 2 or c2.dim() != 2 or c1.shape[0] != self.nbf or c2.shape[0] != self.nbf:
+            raise ValueError(f"C1 and C2 must have {self.nbf} rows")
+        out = torch.empty((self.naux, c1.shape[1], c2.shape[1]), dtype=torch.float64, device=self.device)
+        if out.numel() == 0:
+            return out.cpu().numpy() if (to_np or to_np2) else out
+        c1t = c1.T.contiguous()
+        for P0, P1, X in self._x_blocks(c2):
+            out[P0:P1] = torch.matmul(c1t, X)
+        return out.cpu().numpy() if (to_np or to_np2) else out
+
+    def two_electron_integrals_gen_block(self, C1, C2, C3, C4):
+        """V[i, j, k, l] = <ij|kl> = sum_P B13[P, i, k] B24[P, j, l] with
+        B13 = B_tensor_gen_block(C1, C3), B24 = B_tensor_gen_block(C2, C4)
+        (jkbuilder.py:382-420): one GEMM over P, then (i, k, j, l) -> (i, j, k, l)."""
+        to_np = not all(isinstance(c, torch.Tensor) for c in (C1, C2, C3, C4))
+        same = C2 is C1 and C4 is C3
+        C1, C2, C3, C4 = (self._as_dev(c)[0] for c in (C1, C2, C3, C4))
+        B13 = self.B_tensor_gen_block(C1, C3)
+        B24 = B13 if same else self.B_tensor_gen_block(C2, C4)
+        Q, n1, n3 = B13.shape
+        n2, n4 = B24.shape[1], B24.shape[2]
+        V = (B13.reshape(Q, n1 * n3).T @ B24.reshape(Q, n2 * n4)).reshape(n1, n3, n2, n4).permute(0, 2, 1, 3)
+        V = V.contiguous()
+        return V.cpu().numpy() if to_np else V
+
+    def two_electron_integrals_block(self, C):
+        """<pq|rs> over one orbital set (jkbuilder.py:422-443)."""
+        c, to_np = self._as_dev(C)
+        V = self.two_electron_integrals_gen_block(c, c, c, c)
+        return V.cpu().numpy() if to_np else V
+
     # ------------------------------------------------------------------ measurement
     def work(self, nocc, precision="fp64", want_j=True, want_k=True):
         """(J bytes, K flops) of one build: Bytes_J = 2 Q N(N+1)/2 s_B + 8 nd (2Q + 2N^2),

@@ -161,6 +161,16 @@ int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens,
                         double* const* J, double* const* K, int precision, int flags,
                         void* const* ws, const size_t* ws_bytes, void* stream);
 
+/* Half-transform X[P-p0][m][i] = sum_n B[P][m][n] C[n][i] for p0 <= P < p1,
+ * i < ncol, into the caller's X ((p1-p0)*nbf rows, row stride ldx), FP64.  The first step of
+ * FockBuilder.B_tensor_gen_block (forte2/jkbuilder/jkbuilder.py:521-552),
+ * whose einsum "Pmn,mi,nj->Pij" is finished by a GEMM with C1 on the caller's
+ * side; two_electron_integrals_gen_block (:382-420) contracts two such blocks
+ * over P.  Workspace: f2jk_workspace_size(ctx, 1, &ncol, FP64, F2JK_WANT_K). */
+int f2jk_half_transform(f2jk_ctx* ctx, const double* C, int64_t ncol, int64_t ldc, int64_t p0,
+                        int64_t p1, double* X, int64_t ldx, int precision, void* ws, size_t ws_bytes,
+                        void* stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* f2jk_last_error(void);
 

@@ -80,3 +80,15 @@ def build_active_fock(B, C_act, g1):
 def eri_from_B(B):
     """(mn|rs) = sum_P B_Pmn B_Prs -- brute-force helper for small N."""
     return np.einsum("Pmn,Prs->mnrs", B, B)
+
+
+def B_tensor_gen_block(B, C1, C2):
+    """MO-basis B: B_Pij = sum_mn B_Pmn C1_mi C2_nj (jkbuilder.py:521-552,
+    real orbitals)."""
+    return np.einsum("Pmn,mi,nj->Pij", B, C1, C2, optimize=True)
+
+
+def two_electron_integrals_gen_block(B, C1, C2, C3, C4):
+    """<ij|kl> = sum_P B_P(C1, C3)_ik B_P(C2, C4)_jl (jkbuilder.py:382-420,
+    the einsum "Pmn,Prs,mi,rj,nk,sl->ijkl")."""
+    return np.einsum("Pmn,Prs,mi,rj,nk,sl->ijkl", B, B, C1, C2, C3, C4, optimize=True)

@@ -101,3 +101,31 @@ def test_B_build_reproduces_metric_contraction():
     assert np.abs(lhs - rhs).max() < 1e-12
     with pytest.raises(ValueError):
         ojk.build_B_density_fitting(-M, T)
+
+
+def test_mo_blocks_match_brute_force():
+    # B need not be symmetric here: a swapped m/n or r/s index would show
+    rng = np.random.default_rng(7)
+    Q, N = 3, 4
+    B = rng.standard_normal((Q, N, N))
+    C1, C2, C3, C4 = (rng.standard_normal((N, k)) for k in (2, 3, 1, 4))
+    Bmo = ojk.B_tensor_gen_block(B, C1, C3)
+    for P, i, k in itertools.product(range(Q), range(2), range(1)):
+        ref = sum(B[P, m, n] * C1[m, i] * C3[n, k] for m in range(N) for n in range(N))
+        assert abs(Bmo[P, i, k] - ref) < 1e-12
+    V = ojk.two_electron_integrals_gen_block(B, C1, C2, C3, C4)
+    assert V.shape == (2, 3, 1, 4)
+    for i, j, k, l in itertools.product(range(2), range(3), range(1), range(4)):
+        ref = sum(B[P, m, n] * B[P, r, s] * C1[m, i] * C2[r, j] * C3[n, k] * C4[s, l]
+                  for P in range(Q) for m, n, r, s in itertools.product(range(N), repeat=4))
+        assert abs(V[i, j, k, l] - ref) < 1e-11
+
+
+def test_mo_blocks_physicist_convention():
+    # <pq|rs> = (pr|qs): with C = 1, V is the AO chemist ERI with axes 1 and 2 swapped
+    rng = np.random.default_rng(8)
+    B = _sym_B(rng, 5, 4)
+    eye = np.eye(4)
+    V = ojk.two_electron_integrals_gen_block(B, eye, eye, eye, eye)
+    assert np.abs(V - ojk.eri_from_B(B).transpose(0, 2, 1, 3)).max() < 1e-13
+    assert np.abs(ojk.B_tensor_gen_block(B, eye, eye) - B).max() == 0.0

@@ -87,3 +87,20 @@ def test_duplicate_aux_shells_make_metric_singular():
     aux.add(0, [1.0], [1.0], [0, 0, 0])
     with pytest.raises(ValueError):
         ojk.build_B_density_fitting(ints.coulomb_2c(aux), ints.coulomb_3c(aux, s.basis))
+
+
+def test_mp2_golden():
+    # tests/mp2/test_mp2.py:8-54: DF-MP2 of water in cc-pVQZ / cc-pVQZ-JKFIT from
+    # <ij|ab> = two_electron_integrals_gen_block(Co, Co, Cv, Cv)
+    s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    r = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                     X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+    assert r.E == pytest.approx(-76.0614664072629836, abs=5e-8)
+    no = r.na
+    Co, Cv = r.C[:, :no], r.C[:, no:]
+    V = ojk.two_electron_integrals_gen_block(B, Co, Co, Cv, Cv)
+    eo, ev = r.eps[:no], r.eps[no:]
+    den = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+    emp2 = r.E + np.sum(V * (2.0 * V - V.transpose(0, 1, 3, 2)) / den)
+    assert emp2 == pytest.approx(-76.3710978841482984, abs=5e-8)
