@@ -223,13 +223,17 @@ if __name__ == "__main__":
     ap.add_argument("--grouped-waters", default="2,4,8,12,16,24")
     ap.add_argument("--group", type=int, default=4)
     ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--merge", action="store_true",
+                    help="update the entries of the existing table at --out instead of replacing it")
+    ap.add_argument("--skip-grouped", action="store_true", help="single-job entries only")
     a = ap.parse_args()
     log = lambda m: print(m, flush=True)
     precs = tuple(a.precisions.split(","))
     ints = lambda s: [int(x) for x in s.split(",") if x]
-    table = {}
+    table = load(a.out) if a.merge else {}
     table.update(tune(water_shapes(ints(a.waters)) + alkane_shapes(ints(a.alkanes)), precs, log=log))
-    table.update(tune_grouped(water_shapes(ints(a.grouped_waters)) + alkane_shapes(ints(a.alkanes)), precs,
-                              group=a.group, log=log))
+    if not a.skip_grouped:
+        table.update(tune_grouped(water_shapes(ints(a.grouped_waters)) + alkane_shapes(ints(a.alkanes)), precs,
+                                  group=a.group, log=log))
     save(table, a.out)
     log(f"[autotune] wrote {len(table)} entries to {a.out}")

@@ -326,26 +326,40 @@ class FockBuilder:
         return self.build_core_fock(C_core, hcore, precision) + self.build_active_fock(C_act, g1, precision)
 
     # ------------------------------------------------------------------ MO-basis blocks
-    def _x_blocks(self, C, block_bytes=1 << 30):
-        """Yields (P0, P1, X) with X[P-P0, m, i] = sum_n B_Pmn C_ni (FP64, on the
-        device, f2jk_half_transform) for blocks of P of at most block_bytes."""
+    def half_transform(self, C, p0=0, p1=None, out=None):
+        """X[P - p0, m, i] = sum_n B_Pmn C_ni for p0 <= P < p1, FP64 on the
+        device (f2jk_half_transform, the MFMA kernel of the K build).  C is a
+        device tensor (nbf, ncol); out, if given, a contiguous FP64 device
+        tensor (p1 - p0, nbf, ncol)."""
         lib = nat.jk_lib()
+        p1 = self.naux if p1 is None else int(p1)
         t, ld = self._rowmajor(C)
-        N, Q, ncol = self.nbf, self.naux, int(t.shape[1])
+        N, ncol = self.nbf, int(t.shape[1])
+        if t.shape[0] != N:
+            raise ValueError(f"C must have {N} rows")
+        shape = (p1 - p0, N, ncol)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        elif tuple(out.shape) != shape or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous float64 tensor of shape {shape}")
         nbytes = ctypes.c_size_t()
-        nocc_a = nat.i64_array([ncol])
-        nat.jk_check(lib.f2jk_workspace_size(self.ctx, 1, nocc_a, nat.PREC["fp64"], nat.F2JK_WANT_K,
+        nat.jk_check(lib.f2jk_workspace_size(self.ctx, 1, nat.i64_array([ncol]), nat.PREC["fp64"], nat.F2JK_WANT_K,
                                              ctypes.byref(nbytes)), "workspace")
         ws = self._workspace(nbytes.value)
+        nat.jk_check(lib.f2jk_half_transform(self.ctx, ctypes.c_void_p(t.data_ptr()), ncol, ld, p0, p1,
+                                             ctypes.c_void_p(out.data_ptr()), ncol, nat.PREC["fp64"],
+                                             ctypes.c_void_p(ws.data_ptr()), ws.numel(), self._stream()),
+                     "f2jk_half_transform")
+        return out
+
+    def _x_blocks(self, C, block_bytes=1 << 30):
+        """Yields (P0, P1, X) with X = half_transform(C, P0, P1) for blocks of P
+        of at most block_bytes."""
+        N, Q, ncol = self.nbf, self.naux, int(C.shape[1])
         step = max(1, min(Q, block_bytes // (8 * N * ncol)))
         for P0 in range(0, Q, step):
             P1 = min(Q, P0 + step)
-            X = torch.empty((P1 - P0, N, ncol), dtype=torch.float64, device=self.device)
-            nat.jk_check(lib.f2jk_half_transform(self.ctx, ctypes.c_void_p(t.data_ptr()), ncol, ld, P0, P1,
-                                                 ctypes.c_void_p(X.data_ptr()), ncol, nat.PREC["fp64"],
-                                                 ctypes.c_void_p(ws.data_ptr()), ws.numel(), self._stream()),
-                         "f2jk_half_transform")
-            yield P0, P1, X
+            yield P0, P1, self.half_transform(C, P0, P1)
 
     def B_tensor_gen_block(self, C1, C2):
         """B[P, i, j] = sum_mn B_Pmn C1_mi C2_nj: the MO-basis B tensor

@@ -27,7 +27,8 @@ def canonical_orth(S, rtol=1e-8):
 
 class System:
     def __init__(self, xyz, basis_set, auxiliary_basis_set=None, unit="angstrom",
-                 overlap_ortho_rtol=1e-8, df_ortho_rtol=None, device=None):
+                 overlap_ortho_rtol=1e-8, df_ortho_rtol=None, device=None, jk_mem_thres_mb=None,
+                 integral_backend="auto"):
         self.xyz = xyz
         self.unit = unit
         self.atoms = _geom.parse_geometry(xyz, unit) if isinstance(xyz, str) else list(xyz)
@@ -47,6 +48,8 @@ class System:
         self.Xorth, self.nmo = canonical_orth(self.ints_overlap(), overlap_ortho_rtol)
         self.two_component = False
         self.device = device
+        self.jk_mem_thres_mb = jk_mem_thres_mb
+        self.integral_backend = integral_backend
         self._fock_builder = None
 
     # ------------------------------------------------------------------ integrals
@@ -92,10 +95,28 @@ class System:
     @property
     def fock_builder(self):
         if self._fock_builder is None:
-            from .jkbuilder import FockBuilder
-
-            self._fock_builder = FockBuilder(self, device=self.device)
+            self._fock_builder = self._init_fock_builder()
         return self._fock_builder
+
+    def _init_fock_builder(self):
+        """In-core builder, or the integral-direct one when B and the metric
+        exceed jk_mem_thres_mb (MB of HBM) (forte2/system/system.py:405-434)."""
+        from .jkbuilder import FockBuilder
+
+        if self.jk_mem_thres_mb is None:
+            return FockBuilder(self, device=self.device)
+        if self.auxiliary_basis is None:
+            raise ValueError("If jk_mem_thres_mb is set, auxiliary_basis_set must be provided to determine "
+                             "the size of the B tensor.")
+        b_mb = 8 * self.naux * self.nbf**2 / 1024**2
+        metric_mb = 8 * self.naux**2 / 1024**2
+        if b_mb + metric_mb > self.jk_mem_thres_mb:
+            from .otf import FockBuilderOTF
+
+            return FockBuilderOTF(self, jk_mem_thres_mb=self.jk_mem_thres_mb, backend=self.integral_backend,
+                                  device=self.device)
+        # the reference's third choice (no transposed copy) is the only layout here
+        return FockBuilder(self, store_B_nPm=2 * b_mb + metric_mb <= self.jk_mem_thres_mb, device=self.device)
 
     @fock_builder.setter
     def fock_builder(self, fb):
