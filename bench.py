@@ -30,7 +30,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 # with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size:
 # x2 and x1) on these kernels and launch shapes (profiles/pmc_traffic_h2o64_bf16x3.md);
 # other workloads report null.
-MEASURED_KX_TRAFFIC = {(64, "bf16x3"): 19.50e9 + 3.55e9}
+MEASURED_KX_TRAFFIC = {(64, "bf16x3"): 18.42e9 + 3.64e9}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -51,6 +51,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 4 GiB device copy (x3), the reference for HBM byte counters")
+    ap.add_argument("--otf", type=int, default=None, metavar="N",
+                    help="time the integral-direct builder on (H2O)_N instead (cc-pVDZ/cc-pVTZ-JKFIT)")
+    ap.add_argument("--otf-mem", type=float, default=4000.0, help="jk_mem_thres_mb of the integral-direct builder")
+    ap.add_argument("--otf-cache", type=float, default=16000.0,
+                    help="MB of (P|mn) blocks kept in HBM for the cached leg of --otf")
     ap.add_argument("--serve", choices=["water", "alkane"], default=None,
                     help="instead of the J/K microbenchmark, serve a stream of SCF requests and report "
                          "p50/p99 latency and jobs/s: (H2O)_n RHF cc-pVDZ, or C_nH_(2n+2) (+ cations) UHF cc-pVTZ")
@@ -184,6 +189,79 @@ def serve_mode(args):
     }))
 
 
+def otf_mode(args):
+    """The integral-direct builder (FockBuilderOTF) on a real (H2O)_n geometry:
+    (P|mn) evaluated by the host engine every sweep, contracted on the device.
+    A second leg keeps the (P|mn) blocks resident in HBM (cache)."""
+    import numpy as np
+    import torch
+    from threadpoolctl import threadpool_limits
+
+    from forte2_amd import geometry as geo
+    from forte2_amd import integrals
+    from forte2_amd.otf import FockBuilderOTF
+    from forte2_amd.system import System
+    from forte2_amd.workloads import orthonormal_C
+    from oracle import otf as ootf
+
+    n = args.otf
+    dev = torch.device("cuda", 0)
+    cores = host_cores()
+    integrals.set_threads(cores)
+    s = System(geo.water_cluster(n, 9.0 if n == 16 else None, seed=n), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    N, Q, o = s.nbf, s.naux, s.Zsum // 2
+    C = orthonormal_C(N, o, seed=n, device=dev)
+    print(f"[otf] (H2O)_{n}: N={N} Q={Q} o={o}, {cores} host cores", file=sys.stderr, flush=True)
+
+    def leg(cache_mb):
+        fb = FockBuilderOTF(s, jk_mem_thres_mb=args.otf_mem, cache_mb=cache_mb, device=dev)
+        for _ in range(args.warmup):
+            fb.build_JK([C])
+        torch.cuda.synchronize()
+        h0 = fb.host_seconds
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            J, K = fb.build_JK([C])
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        info = {"ms_per_build": t * 1e3, "builds_per_s": 1.0 / t,
+                "host_integral_s_per_build": (fb.host_seconds - h0) / args.steps,
+                "blocks": len(fb.blocks), "pblksize": fb.pblksize, "iblksize": fb.iblksize}
+        fb.release()
+        return info, J[0], K[0]
+
+    plain, J, K = leg(0.0)
+    print(f"[otf] direct: {plain['ms_per_build']:.1f} ms/build", file=sys.stderr, flush=True)
+    cached, Jc, Kc = leg(args.otf_cache)
+    print(f"[otf] cached: {cached['ms_per_build']:.1f} ms/build", file=sys.stderr, flush=True)
+    sweeps = -(-o // plain["iblksize"])
+    # CPU: the same host integrals per sweep plus the oracle's numpy contractions
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        T = s.coulomb_3c()
+        t_int = time.perf_counter() - t0
+        met = ootf.Metric(s.coulomb_2c())
+        Ch = C.cpu().numpy()
+        blocks = [(0, Q)]
+        t0 = time.perf_counter()
+        Jr, Kr = ootf.JK_kernel(met, T, blocks, Ch, plain["iblksize"])
+        t_con = time.perf_counter() - t0
+    t_cpu = (sweeps + (1 if sweeps == 1 else 0)) * t_int + t_con
+    err = max(float(np.abs(J.cpu().numpy() - Jr).max()), float(np.abs(K.cpu().numpy() - Kr).max()))
+    print(json.dumps({
+        "metric": "integral-direct JK builds/s", "value": plain["builds_per_s"], "unit": "builds/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": plain["ms_per_build"],
+        "higher_is_better": True, "dtype": "fp64", "data": "synthetic geometry",
+        "config": {"workload": f"(H2O)_{n} RHF J/K build, integral-direct, cc-pVDZ/cc-pVTZ-JKFIT "
+                               f"(nbf {N}, naux {Q}, nocc {o}), jk_mem_thres_mb {args.otf_mem}"},
+        "direct": plain, "cached": dict(cached, cache_mb=args.otf_cache),
+        "max_abs_diff_vs_oracle": err,
+        "cpu_baseline": {"value": 1.0 / t_cpu, "unit": "builds/s", "cores": cores, "kind": "port",
+                         "sample": f"full build: {sweeps + (1 if sweeps == 1 else 0)} host evaluations of (P|mn) "
+                                   f"({t_int:.2f} s each) + oracle numpy contractions ({t_con:.2f} s)"},
+    }))
+
+
 def timed_builds(fb, C, prec, steps, warmup):
     import torch
 
@@ -260,6 +338,9 @@ def main():
         return
     if args.serve:
         serve_mode(args)
+        return
+    if args.otf is not None:
+        otf_mode(args)
         return
     if args.copy_calibration:
         copy_calibration()

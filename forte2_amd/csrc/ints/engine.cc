@@ -551,9 +551,61 @@ const std::vector<std::pair<int, int>>& ket_pairs(const f2ints_basis* cbs) {
     return bs->pair_index;
 }
 
-int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1, double* out) {
+// scratch of one thread for eval_3c
+struct Scratch3c {
+    std::vector<double> R, work, Y, acc;
+    Scratch3c(int maxlaux, int maxl) {
+        const int Lmax = maxlaux + 2 * maxl;
+        R.resize(nherm(Lmax));
+        work.resize((size_t)(Lmax + 1) * nherm(Lmax));
+        Y.resize((size_t)nsph(maxlaux) * nherm(2 * maxl));
+    }
+};
+
+// (P|ab) of aux shell P and significant pair q of bs into w.acc[sP][sa * nsb + sb]
+void eval_3c(const f2ints_basis* bs, size_t q, const Shell& P, Scratch3c& w) {
     const Harmonics& Hm = harm();
     const HermIndex& H = hidx();
+    const Shell& A = bs->shells[bs->pair_index[q].first];
+    const Shell& B = bs->shells[bs->pair_index[q].second];
+    const ShellPair& sp = *static_cast<const ShellPair*>(bs->pair_data[q]);
+    const int lab = A.l + B.l, nhab = nherm(lab), nab = nsph(A.l) * nsph(B.l);
+    const int lP = P.l, nsP = nsph(lP), L = lP + lab;
+    w.acc.assign((size_t)nsP * nab, 0.0);
+    for (size_t k = 0; k < P.alpha.size(); ++k) {
+        const double g = P.alpha[k];
+        for (const PrimPair& pp : sp.prims) {
+            const double p = pp.p, al = g * p / (g + p);
+            compute_R(L, al, P.A[0] - pp.P[0], P.A[1] - pp.P[1], P.A[2] - pp.P[2], w.R.data(), w.work.data());
+            const double pref = 2.0 * std::pow(PI, 2.5) / (g * p * std::sqrt(g + p)) * P.coef[k] *
+                                std::pow(2.0 * g, -lP) * pp.scale;
+            // Y[sP][h] = sum_c T_{lP}(sP, c) R[c + h]
+            for (int sP = 0; sP < nsP; ++sP) {
+                double* y = &w.Y[(size_t)sP * nhab];
+                std::fill(y, y + nhab, 0.0);
+                for (const SphTerm& tp : Hm.sparse[lP][sP]) {
+                    const CartComp& cp = Hm.cart[lP][tp.cart];
+                    for (int h = 0; h < nhab; ++h) {
+                        const auto& tuv = H.list[lab][h];
+                        y[h] += tp.c * w.R[H.at(cp.x + tuv[0], cp.y + tuv[1], cp.z + tuv[2])];
+                    }
+                }
+            }
+            for (int sP = 0; sP < nsP; ++sP) {
+                const double* y = &w.Y[(size_t)sP * nhab];
+                double* a = &w.acc[(size_t)sP * nab];
+                for (int s2 = 0; s2 < nab; ++s2) {
+                    const double* e = &pp.E[(size_t)s2 * nhab];
+                    double v = 0.0;
+                    for (int h = 0; h < nhab; ++h) v += e[h] * y[h];
+                    a[s2] += pref * v;
+                }
+            }
+        }
+    }
+}
+
+int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1, double* out) {
     const int64_t nbf = bs->nbf;
     const int64_t first = aux->shells[sh0].off;
     const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
@@ -561,65 +613,58 @@ int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1
     std::fill(out, out + (size_t)nP * nbf * nbf, 0.0);
     int maxlaux = 0;
     for (int s = sh0; s < sh1; ++s) maxlaux = std::max(maxlaux, aux->shells[s].l);
-    const int Lmax = maxlaux + 2 * bs->maxl;
 #pragma omp parallel num_threads(nthreads())
     {
-        std::vector<double> R(nherm(Lmax)), work((size_t)(Lmax + 1) * nherm(Lmax));
-        std::vector<double> Y((size_t)nsph(maxlaux) * nherm(2 * bs->maxl));
-        std::vector<double> acc;
+        Scratch3c w(maxlaux, bs->maxl);
 #pragma omp for schedule(dynamic)
         for (size_t q = 0; q < pairs.size(); ++q) {
             const Shell& A = bs->shells[pairs[q].first];
             const Shell& B = bs->shells[pairs[q].second];
-            const ShellPair& sp = *static_cast<const ShellPair*>(bs->pair_data[q]);
-            const int lab = A.l + B.l, nhab = nherm(lab), nsa = nsph(A.l), nsbb = nsph(B.l);
-            const int nab = nsa * nsbb;
+            const int nsa = nsph(A.l), nsbb = nsph(B.l), nab = nsa * nsbb;
             for (int s = sh0; s < sh1; ++s) {
                 const Shell& P = aux->shells[s];
-                const int lP = P.l, nsP = nsph(lP), L = lP + lab, nhL = nherm(L);
-                acc.assign((size_t)nsP * nab, 0.0);
-                for (size_t k = 0; k < P.alpha.size(); ++k) {
-                    const double g = P.alpha[k];
-                    for (const PrimPair& pp : sp.prims) {
-                        const double p = pp.p, al = g * p / (g + p);
-                        compute_R(L, al, P.A[0] - pp.P[0], P.A[1] - pp.P[1], P.A[2] - pp.P[2], R.data(),
-                                  work.data());
-                        const double pref = 2.0 * std::pow(PI, 2.5) / (g * p * std::sqrt(g + p)) *
-                                            P.coef[k] * std::pow(2.0 * g, -lP) * pp.scale;
-                        // Y[sP][h] = sum_c T_{lP}(sP, c) R[c + h]
-                        for (int sP = 0; sP < nsP; ++sP) {
-                            double* y = &Y[(size_t)sP * nhab];
-                            std::fill(y, y + nhab, 0.0);
-                            for (const SphTerm& tp : Hm.sparse[lP][sP]) {
-                                const CartComp& cp = Hm.cart[lP][tp.cart];
-                                for (int h = 0; h < nhab; ++h) {
-                                    const auto& tuv = H.list[lab][h];
-                                    y[h] += tp.c * R[H.at(cp.x + tuv[0], cp.y + tuv[1], cp.z + tuv[2])];
-                                }
-                            }
-                        }
-                        (void)nhL;
-                        for (int sP = 0; sP < nsP; ++sP) {
-                            const double* y = &Y[(size_t)sP * nhab];
-                            double* a = &acc[(size_t)sP * nab];
-                            for (int s2 = 0; s2 < nab; ++s2) {
-                                const double* e = &pp.E[(size_t)s2 * nhab];
-                                double v = 0.0;
-                                for (int h = 0; h < nhab; ++h) v += e[h] * y[h];
-                                a[s2] += pref * v;
-                            }
-                        }
-                    }
-                }
-                for (int sP = 0; sP < nsP; ++sP) {
+                eval_3c(bs, q, P, w);
+                for (int sP = 0; sP < nsph(P.l); ++sP) {
                     double* o = out + (size_t)(P.off + sP - first) * nbf * nbf;
                     for (int sa = 0; sa < nsa; ++sa)
                         for (int sb = 0; sb < nsbb; ++sb) {
-                            const double v = acc[(size_t)sP * nab + sa * nsbb + sb];
+                            const double v = w.acc[(size_t)sP * nab + sa * nsbb + sb];
                             const int64_t m = A.off + sa, n = B.off + sb;
                             o[m * nbf + n] = v;
                             o[n * nbf + m] = v;
                         }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// (P|ab) for every aux function and the significant pairs [q0, q1), packed:
+// out[P * ldo + c], column c running over the pairs in order, nsa * nsb
+// columns each (row-major in (sa, sb)).
+int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1, double* out,
+                     int64_t ldo) {
+    const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
+    if (q0 < 0 || q1 > (int64_t)pairs.size() || q0 > q1) return fail("f2ints_coulomb_3c_pairs: bad pair range");
+    std::vector<int64_t> col(q1 - q0 + 1, 0);
+    for (int64_t q = q0; q < q1; ++q)
+        col[q - q0 + 1] = col[q - q0] + (int64_t)nsph(bs->shells[pairs[q].first].l) * nsph(bs->shells[pairs[q].second].l);
+    if (ldo < col.back()) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the block's columns");
+    int maxlaux = 0;
+    for (const Shell& P : aux->shells) maxlaux = std::max(maxlaux, P.l);
+#pragma omp parallel num_threads(nthreads())
+    {
+        Scratch3c w(maxlaux, bs->maxl);
+#pragma omp for schedule(dynamic)
+        for (int64_t q = q0; q < q1; ++q) {
+            const int nab = nsph(bs->shells[pairs[q].first].l) * nsph(bs->shells[pairs[q].second].l);
+            double* o = out + col[q - q0];
+            for (const Shell& P : aux->shells) {
+                eval_3c(bs, (size_t)q, P, w);
+                for (int sP = 0; sP < nsph(P.l); ++sP) {
+                    double* row = o + (size_t)(P.off + sP) * ldo;
+                    std::copy(&w.acc[(size_t)sP * nab], &w.acc[(size_t)sP * nab] + nab, row);
                 }
             }
         }
@@ -775,6 +820,28 @@ int f2ints_coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sb, i
     if (sb < 0 || sb > se) return fail("f2ints_coulomb_3c: bad shell range");
     if (sb == se) return 0;
     return coulomb_3c(aux, bs, sb, se, out);
+}
+
+int64_t f2ints_ket_pairs(const f2ints_basis* bs, int* a, int* b, int64_t cap) {
+    if (!bs) {
+        fail("f2ints_ket_pairs: null basis");
+        return -1;
+    }
+    const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
+    const int64_t n = (int64_t)pairs.size();
+    if (a && b)
+        for (int64_t q = 0; q < std::min(n, cap); ++q) {
+            a[q] = pairs[q].first;
+            b[q] = pairs[q].second;
+        }
+    return n;
+}
+
+int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1,
+                            double* out, int64_t ldo) {
+    if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_pairs: null argument");
+    if (q0 == q1) return 0;
+    return coulomb_3c_pairs(aux, bs, q0, q1, out, ldo);
 }
 
 int f2ints_coulomb_4c(const f2ints_basis* b, double* out) {

@@ -34,6 +34,7 @@ accepted and rejected.
 
 import concurrent.futures
 import math
+import time
 
 import numpy as np
 import torch
@@ -135,6 +136,7 @@ class FockBuilderOTF:
         self.cache_bytes = int(cache_mb * 1024**2)
         self._cache = {}
         self.host_blocks_computed = 0  # blocks evaluated by the host engine so far
+        self.host_seconds = 0.0        # wall time of those evaluations (worker thread)
 
     # ------------------------------------------------------------------ metric
     def _build_metric(self):
@@ -163,7 +165,9 @@ class FockBuilderOTF:
     # ------------------------------------------------------------------ (P|mn) blocks
     def _host_block(self, k, slot):
         s0, s1, p0, p1 = self.blocks[k]
+        t0 = time.perf_counter()
         _ints.coulomb_3c(self.auxbasis, self.basis, s0, s1, out=self._host[slot][: p1 - p0].numpy())
+        self.host_seconds += time.perf_counter() - t0
 
     def _sweep(self):
         """Yields (p0, p1, T) for every aux-shell block, T = (P|mn) on the

@@ -44,11 +44,11 @@ def J_kernel(met, T, blocks, D):
     """Two passes over the blocks (jkbuilder.py:813-840)."""
     bP = np.zeros(T.shape[0])
     for p0, p1 in blocks:
-        bP[p0:p1] = np.einsum("Pmn,nm->P", T[p0:p1], D)
+        bP[p0:p1] = np.einsum("Pmn,nm->P", T[p0:p1], D, optimize=True)
     bP = met.apply_Mm1(bP)
     J = np.zeros_like(D)
     for p0, p1 in blocks:
-        J += np.einsum("Pmn,P->mn", T[p0:p1], bP[p0:p1])
+        J += np.einsum("Pmn,P->mn", T[p0:p1], bP[p0:p1], optimize=True)
     return J
 
 
@@ -61,9 +61,9 @@ def K_kernel(met, T, blocks, C, iblk):
         c = C[:, i0:i0 + iblk]
         X = np.zeros((Q, N, c.shape[1]))
         for p0, p1 in blocks:
-            X[p0:p1] = np.einsum("Qms,si->Qmi", T[p0:p1], c)
-        Y = np.einsum("PQ,Qmi->Pmi", met.Mm12, X)
-        K += np.einsum("Pmi,Pni->mn", Y, Y)
+            X[p0:p1] = np.einsum("Qms,si->Qmi", T[p0:p1], c, optimize=True)
+        Y = np.einsum("PQ,Qmi->Pmi", met.Mm12, X, optimize=True)
+        K += np.einsum("Pmi,Pni->mn", Y, Y, optimize=True)
     return K
 
 
@@ -80,19 +80,19 @@ def JK_kernel(met, T, blocks, C, iblk):
         c = C[:, i0:i0 + iblk]
         X = np.zeros((Q, N, c.shape[1]))
         for p0, p1 in blocks:
-            X[p0:p1] = np.einsum("Qms,si->Qmi", T[p0:p1], c)
+            X[p0:p1] = np.einsum("Qms,si->Qmi", T[p0:p1], c, optimize=True)
             if jpass == 0:
-                bP[p0:p1] = np.einsum("Pmn,nm->P", T[p0:p1], D)
+                bP[p0:p1] = np.einsum("Pmn,nm->P", T[p0:p1], D, optimize=True)
             elif jpass == 1:
-                J += np.einsum("Pmn,P->mn", T[p0:p1], bP[p0:p1])
-        Y = np.einsum("PQ,Qmi->Pmi", met.Mm12, X)
-        K += np.einsum("Pmi,Pni->mn", Y, Y)
+                J += np.einsum("Pmn,P->mn", T[p0:p1], bP[p0:p1], optimize=True)
+        Y = np.einsum("PQ,Qmi->Pmi", met.Mm12, X, optimize=True)
+        K += np.einsum("Pmi,Pni->mn", Y, Y, optimize=True)
         if jpass == 0:
             bP = met.apply_Mm1(bP)
         jpass += 1
     if jpass == 1:
         for p0, p1 in blocks:
-            J += np.einsum("Pmn,P->mn", T[p0:p1], bP[p0:p1])
+            J += np.einsum("Pmn,P->mn", T[p0:p1], bP[p0:p1], optimize=True)
     return J, K
 
 
