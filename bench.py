@@ -99,13 +99,21 @@ def scf_mode(args):
     t0 = time.perf_counter()
     s = System(geo.water_cluster(n, box, seed=seed), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
     fb = s.fock_builder
-    fb.progress = lambda msg: print(f"[scf] {msg}", file=sys.stderr, flush=True)
+    last = [0.0]
+
+    def progress(msg):
+        if time.perf_counter() - last[0] > 5.0:
+            last[0] = time.perf_counter()
+            print(f"[scf] {msg}", file=sys.stderr, flush=True)
+
+    fb.progress = progress
     fb.B_Pmn
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
     print(f"[scf] (H2O)_{n} N={s.nbf} Q={s.naux}: B formed in {t_b:.1f} s", file=sys.stderr, flush=True)
     out = {"workload": f"(H2O)_{n} RHF cc-pVDZ/cc-pVTZ-JKFIT, box {box} A, seed {seed}",
-           "nbf": s.nbf, "naux": s.naux, "nocc": s.Zsum // 2, "B_formation_s": t_b, "e_tol": 1e-10}
+           "nbf": s.nbf, "naux": s.naux, "nocc": s.Zsum // 2, "B_formation_s": t_b,
+           "B_formation_detail": fb.B_stats, "e_tol": 1e-10}
     runs = {}
     for prec in ("fp64", args.mixed):
         fb.timing = False

@@ -13,6 +13,8 @@ F2INTS_SYMBOLS = (
     "f2ints_nuclear_repulsion",
     "f2ints_coulomb_2c",
     "f2ints_coulomb_3c",
+    "f2ints_ket_pairs",
+    "f2ints_coulomb_3c_pairs",
     "f2ints_coulomb_4c",
     "f2ints_boys",
     "f2ints_cart2sph",
@@ -36,6 +38,9 @@ def declare(lib):
     lib.f2ints_nuclear_repulsion.restype = D
     lib.f2ints_coulomb_2c.argtypes = [V, V, Dp]
     lib.f2ints_coulomb_3c.argtypes = [V, V, I, I, Dp]
+    lib.f2ints_ket_pairs.argtypes = [V, Ip, Ip, I64]
+    lib.f2ints_ket_pairs.restype = I64
+    lib.f2ints_coulomb_3c_pairs.argtypes = [V, V, I64, I64, Dp, I64]
     lib.f2ints_coulomb_4c.argtypes = [V, Dp]
     lib.f2ints_boys.argtypes = [I, D, Dp]
     lib.f2ints_cart2sph.argtypes = [I, Dp]
@@ -43,5 +48,5 @@ def declare(lib):
     lib.f2ints_last_error.argtypes = []
     lib.f2ints_last_error.restype = ctypes.c_char_p
     for s in F2INTS_SYMBOLS:
-        if s not in ("f2ints_basis_nbf", "f2ints_nuclear_repulsion", "f2ints_last_error"):
+        if s not in ("f2ints_basis_nbf", "f2ints_nuclear_repulsion", "f2ints_last_error", "f2ints_ket_pairs"):
             getattr(lib, s).restype = I

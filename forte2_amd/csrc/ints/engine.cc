@@ -640,16 +640,23 @@ int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1
     return 0;
 }
 
+// packed columns of pair q: nsa * nsb, or nsa (nsa + 1) / 2 for a == b
+int64_t pair_ncols(const f2ints_basis* bs, const std::pair<int, int>& ab) {
+    const int na = nsph(bs->shells[ab.first].l), nb = nsph(bs->shells[ab.second].l);
+    return ab.first == ab.second ? (int64_t)na * (na + 1) / 2 : (int64_t)na * nb;
+}
+
 // (P|ab) for every aux function and the significant pairs [q0, q1), packed:
-// out[P * ldo + c], column c running over the pairs in order, nsa * nsb
-// columns each (row-major in (sa, sb)).
+// out[P * ldo + c], column c running over the pairs in order; a pair a > b
+// has nsa * nsb columns (row-major in (sa, sb)), a pair a == b the
+// nsa (nsa + 1) / 2 components sa >= sb (row-major), so every (m, n) with
+// m >= n appears once.
 int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1, double* out,
                      int64_t ldo) {
     const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
     if (q0 < 0 || q1 > (int64_t)pairs.size() || q0 > q1) return fail("f2ints_coulomb_3c_pairs: bad pair range");
     std::vector<int64_t> col(q1 - q0 + 1, 0);
-    for (int64_t q = q0; q < q1; ++q)
-        col[q - q0 + 1] = col[q - q0] + (int64_t)nsph(bs->shells[pairs[q].first].l) * nsph(bs->shells[pairs[q].second].l);
+    for (int64_t q = q0; q < q1; ++q) col[q - q0 + 1] = col[q - q0] + pair_ncols(bs, pairs[q]);
     if (ldo < col.back()) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the block's columns");
     int maxlaux = 0;
     for (const Shell& P : aux->shells) maxlaux = std::max(maxlaux, P.l);
@@ -658,13 +665,22 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0
         Scratch3c w(maxlaux, bs->maxl);
 #pragma omp for schedule(dynamic)
         for (int64_t q = q0; q < q1; ++q) {
-            const int nab = nsph(bs->shells[pairs[q].first].l) * nsph(bs->shells[pairs[q].second].l);
+            const int na = nsph(bs->shells[pairs[q].first].l), nb = nsph(bs->shells[pairs[q].second].l);
+            const int nab = na * nb;
+            const bool diag = pairs[q].first == pairs[q].second;
             double* o = out + col[q - q0];
             for (const Shell& P : aux->shells) {
                 eval_3c(bs, (size_t)q, P, w);
                 for (int sP = 0; sP < nsph(P.l); ++sP) {
                     double* row = o + (size_t)(P.off + sP) * ldo;
-                    std::copy(&w.acc[(size_t)sP * nab], &w.acc[(size_t)sP * nab] + nab, row);
+                    const double* a = &w.acc[(size_t)sP * nab];
+                    if (!diag) {
+                        std::copy(a, a + nab, row);
+                    } else {
+                        int c = 0;
+                        for (int sa = 0; sa < na; ++sa)
+                            for (int sb = 0; sb <= sa; ++sb) row[c++] = a[sa * nb + sb];
+                    }
                 }
             }
         }

@@ -2,10 +2,10 @@
 
 forte2 forms B on the host (forte2/jkbuilder/jkbuilder.py:119-159):
 M = (P|Q), L = chol(M), X = L^-1, B = X (Q|mn) with an einsum.  Here the
-3-centre integrals are evaluated by the host engine one aux-shell block at a
-time and copied into the B tensor in HBM, then B = L^-1 (Q|mn) is applied in
-place on the device by column blocks (triangular solve), so the host never
-holds more than one block of (Q|mn).
+3-centre integrals are evaluated by the host engine one block of basis-shell
+pairs (m >= n) at a time, for all Q, and each block is multiplied by X on
+the device as soon as it arrives (host evaluation, copy and GEMM overlap);
+the host never holds more than two blocks of (Q|mn).
 
 With ``df_ortho_rtol`` set, X = M^-1/2 with eigenvalues below rtol*max
 discarded (forte2/helpers/matrix_functions.py:52-92), as in the reference.
@@ -49,44 +49,125 @@ def _fitting_matrix(kind, fac, blk=1024):
     return X
 
 
-def build_B_density_fitting(system, device, block_bytes=1 << 30, solve_bytes=1 << 30, progress=None):
-    """B (naux, nbf, nbf) FP64 on `device`; `progress(msg)` is called per host block."""
+def pair_columns(basis):
+    """Column layout of the packed unique half of (P|mn) written by
+    f2ints_coulomb_3c_pairs: the significant shell pairs (a >= b) in order, a
+    pair a > b with nsa * nsb columns (row-major), a pair a == b with its
+    components sa >= sb (row-major).  Returns (col0, m_of_col, n_of_col):
+    col0[q] is the first column of pair q (length npairs + 1), and m >= n are
+    the basis functions of every column."""
+    import numpy as np
+
+    a, b = _ints.ket_pairs(basis)
+    offs = np.asarray(basis.shell_offsets(), dtype=np.int64)
+    size = offs[1:] - offs[:-1]
+    cols_m, cols_n, counts = [], [], []
+    # the component pattern of each (la, lb, diagonal) kind, built once
+    pattern = {}
+    for A, Bs in zip(a, b):
+        key = (int(size[A]), int(size[Bs]), bool(A == Bs))
+        if key not in pattern:
+            na, nb, diag = key
+            sa, sb = np.meshgrid(np.arange(na), np.arange(nb), indexing="ij")
+            keep = (sa >= sb) if diag else np.ones_like(sa, dtype=bool)
+            pattern[key] = (sa[keep], sb[keep])
+        sa, sb = pattern[key]
+        cols_m.append(offs[A] + sa)
+        cols_n.append(offs[Bs] + sb)
+        counts.append(len(sa))
+    col0 = np.zeros(len(a) + 1, dtype=np.int64)
+    np.cumsum(counts, out=col0[1:])
+    return col0, np.concatenate(cols_m), np.concatenate(cols_n)
+
+
+def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, stats=None):
+    """B (naux, nbf, nbf) FP64 on `device`, B = X (Q|mn) with X = L^-1 (or
+    M^-1/2), streamed by blocks of basis-shell pairs.
+
+    The host engine evaluates only the unique half (m >= n shell pairs) of
+    (Q|mn), one block of pairs at a time for all Q, on a worker thread into
+    one of two pinned staging buffers.  Meanwhile the device copies the
+    previous block in, multiplies it by X (a (Q, Q) x (Q, ncols) GEMM, half
+    the flops of the full square) and scatters the columns to (m, n) and
+    (n, m) of B.  `stats`, if a dict, receives the total seconds, the
+    seconds the host engine spent (overlapped with the device work), the
+    number of blocks and of packed columns."""
+    import concurrent.futures
+    import time
+
+    import numpy as np
+
     aux, bas = system.auxiliary_basis, system.basis
     if aux is None:
         raise ValueError("Auxiliary basis is not defined. Define auxiliary_basis_set.")
     Q, N = aux.size, bas.size
+    t_start = time.perf_counter()
     M = torch.from_numpy(_ints.coulomb_2c(aux)).to(device)
     kind, fac = metric_factor(M, getattr(system, "df_ortho_rtol", None))
-    B = torch.empty((Q, N, N), dtype=torch.float64, device=device)
-    # stream (Q|mn) into B by aux-shell blocks
-    offs = aux.shell_offsets()
-    per_fn = N * N * 8
-    s0 = 0
-    host = None
-    while s0 < aux.nshells:
-        s1 = s0 + 1
-        while s1 < aux.nshells and (offs[s1 + 1] - offs[s0]) * per_fn <= block_bytes:
-            s1 += 1
-        nP = offs[s1] - offs[s0]
-        if host is None or host.shape[0] < nP:
-            host = torch.empty((nP, N, N), dtype=torch.float64, pin_memory=True)
-        blk = host[:nP]
-        _ints.coulomb_3c(aux, bas, s0, s1, out=blk.numpy())
-        B[offs[s0]:offs[s1]].copy_(blk, non_blocking=False)
-        if progress is not None:
-            progress(f"(Q|mn) aux functions {offs[s1]}/{Q} on device")
-        s0 = s1
     X = _fitting_matrix(kind, fac)
-    # B = X T in place, one contiguous column chunk of the (Q, N*N) view at a time
+    del M, fac
+    col0, mcol, ncol_ = pair_columns(bas)
+    npairs, ncols = len(col0) - 1, int(col0[-1])
+    # pair blocks of at most block_bytes of packed columns
+    width = max(1, block_bytes // (8 * Q))
+    blocks, q = [], 0
+    while q < npairs:
+        q1 = q + 1
+        while q1 < npairs and col0[q1 + 1] - col0[q] <= width:
+            q1 += 1
+        blocks.append((q, q1, int(col0[q]), int(col0[q1])))
+        q = q1
+    B = torch.zeros((Q, N, N), dtype=torch.float64, device=device)
+    if not blocks:
+        return B
+    wmax = max(c1 - c0 for _, _, c0, c1 in blocks)
+    idx_mn = torch.from_numpy(mcol * N + ncol_).to(device)
+    idx_nm = torch.from_numpy(ncol_ * N + mcol).to(device)
     Bv = B.view(Q, N * N)
-    chunk = max(1, min(N * N, solve_bytes // (8 * Q)))
-    for c0 in range(0, N * N, chunk):
-        c1 = min(N * N, c0 + chunk)
-        blk = Bv[:, c0:c1].contiguous()
-        Bv[:, c0:c1] = X @ blk
-        del blk
-    return B
+    pin = torch.device(device).type == "cuda"
+    host = [torch.empty(Q * wmax, dtype=torch.float64, pin_memory=pin) for _ in range(2)]
+    dev_in = torch.empty(Q * wmax, dtype=torch.float64, device=device)
+    host_s = [0.0]
 
+    def packed(flat, w):
+        return flat[: Q * w].view(Q, w)  # contiguous (Q, w) block of a flat buffer
+
+    def evaluate(k, slot):
+        q0, q1, c0, c1 = blocks[k]
+        t0 = time.perf_counter()
+        _ints.coulomb_3c_pairs(aux, bas, q0, q1, packed(host[slot], c1 - c0).numpy())
+        host_s[0] += time.perf_counter() - t0
+
+    stream = torch.cuda.current_stream(device) if pin else None
+    copied = [None, None]
+    with concurrent.futures.ThreadPoolExecutor(max_workers=1) as pool:
+        fut = pool.submit(evaluate, 0, 0)
+        for k, (q0, q1, c0, c1) in enumerate(blocks):
+            slot = k % 2
+            fut.result()
+            if k + 1 < len(blocks):
+                nxt = (k + 1) % 2
+                if copied[nxt] is not None:
+                    copied[nxt].synchronize()
+                fut = pool.submit(evaluate, k + 1, nxt)
+            T = packed(dev_in, c1 - c0)
+            T.copy_(packed(host[slot], c1 - c0), non_blocking=pin)
+            if pin:
+                ev = torch.cuda.Event()
+                ev.record(stream)
+                copied[slot] = ev
+            Bp = X @ T
+            Bv.index_copy_(1, idx_nm[c0:c1], Bp)
+            Bv.index_copy_(1, idx_mn[c0:c1], Bp)
+            del Bp
+            if progress is not None:
+                progress(f"B: shell pairs {q1}/{npairs} ({c1}/{ncols} packed columns)")
+    if pin:
+        torch.cuda.synchronize(device)
+    if stats is not None:
+        stats.update(total_s=time.perf_counter() - t_start, host_s=host_s[0], blocks=len(blocks),
+                     packed_columns=ncols)
+    return B
 
 
 def host_df_integrals(system):

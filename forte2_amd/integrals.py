@@ -77,6 +77,29 @@ def coulomb_3c(aux, basis, shell_begin=0, shell_end=None, out=None):
     return out
 
 
+def ket_pairs(basis):
+    """(a, b) int arrays of the significant shell pairs a >= b, in the order
+    of coulomb_3c_pairs."""
+    lib = _lib()
+    n = lib.f2ints_ket_pairs(basis.handle, None, None, 0)
+    if n < 0:
+        raise RuntimeError(lib.f2ints_last_error().decode())
+    a = np.empty(n, dtype=np.int32)
+    b = np.empty(n, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.f2ints_ket_pairs(basis.handle, a.ctypes.data_as(ip), b.ctypes.data_as(ip), n)
+    return a, b
+
+
+def coulomb_3c_pairs(aux, basis, q0, q1, out):
+    """(P|ab) of all aux functions and the pairs [q0, q1) of ket_pairs,
+    packed into out[:, :ncols] (out: (naux, ldo) C-contiguous float64)."""
+    if out.ndim != 2 or out.shape[0] != aux.size or not out.flags.c_contiguous or out.dtype != np.float64:
+        raise ValueError("out has the wrong shape/layout")
+    _check(_lib().f2ints_coulomb_3c_pairs(aux.handle, basis.handle, int(q0), int(q1), _ptr(out), out.shape[1]))
+    return out
+
+
 def coulomb_4c(basis):
     n = basis.size
     out = np.empty((n, n, n, n))

@@ -64,6 +64,7 @@ class FockBuilder:
         self.last_timings = None
         self.last_launches = None
         self.progress = None
+        self.B_stats = None  # timings of the lazy B formation
         if precision not in nat.PREC:
             raise ValueError(f"unknown precision {precision!r}")
         self.precision = precision
@@ -93,7 +94,9 @@ class FockBuilder:
         if self._B64 is None:
             from .df import build_B_density_fitting
 
-            self._set_B(build_B_density_fitting(self.system, device=self.device, progress=self.progress))
+            self.B_stats = {}
+            self._set_B(build_B_density_fitting(self.system, device=self.device, progress=self.progress,
+                                                stats=self.B_stats))
         return self._B64
 
     def B_fp32(self):

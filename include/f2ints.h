@@ -61,6 +61,20 @@ int f2ints_coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* M);
 int f2ints_coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin,
                       int shell_end, double* out);
 
+/* The significant shell pairs (a >= b) of `bs`, in the order the 3-centre
+ * routines use: returns their number and, when a and b are not NULL, writes
+ * the first min(count, cap) of them.  -1 on error.  Used to stream B by
+ * pair blocks (the mn-blocks of forte2/jkbuilder/jkbuilder.py:135-159). */
+int64_t f2ints_ket_pairs(const f2ints_basis* bs, int* a, int* b, int64_t cap);
+
+/* (P|ab) for every aux function and the significant pairs [pair_begin,
+ * pair_end), packed: out[P * ldo + c], where c runs over the pairs in order.
+ * A pair a > b has nsph(la) * nsph(lb) columns (row-major in the two shells'
+ * components); a pair a == b has the n (n + 1) / 2 components sa >= sb
+ * (row-major), n = nsph(la).  So every (m, n) with m >= n appears once. */
+int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t pair_begin,
+                            int64_t pair_end, double* out, int64_t ldo);
+
 /* (mn|rs) chemist's notation, nbf^4 -- small bases only (tests). */
 int f2ints_coulomb_4c(const f2ints_basis* b, double* out);
 

@@ -146,3 +146,27 @@ def test_contraction_normalisation():
     for name in ("sto-3g", "cc-pvdz", "cc-pvqz"):
         S = ints.overlap(bs.build_basis(name, atoms))
         assert np.abs(np.diag(S) - 1.0).max() < 1e-13
+
+
+def test_three_center_pair_blocks_match_square():
+    # the packed unique half (f2ints_coulomb_3c_pairs) holds exactly the
+    # (P|mn), m >= n, of the square routine, block by block
+    from forte2_amd.df import pair_columns
+
+    atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5")
+    aux, b = bs.build_basis("cc-pvtz-jkfit", atoms), bs.build_basis("cc-pvdz", atoms)
+    full = ints.coulomb_3c(aux, b)
+    col0, m, n = pair_columns(b)
+    N = b.size
+    assert np.all(m >= n) and len(col0) - 1 == len(ints.ket_pairs(b)[0])
+    # every unique (m, n) appears once
+    assert len(set(zip(m.tolist(), n.tolist()))) == len(m) == N * (N + 1) // 2
+    npairs = len(col0) - 1
+    for q0, q1 in ((0, npairs), (0, 5), (5, 17), (17, npairs)):
+        c0, c1 = col0[q0], col0[q1]
+        out = np.full((aux.size, c1 - c0 + 3), np.nan)
+        ints.coulomb_3c_pairs(aux, b, q0, q1, out)
+        assert np.array_equal(out[:, : c1 - c0], full[:, m[c0:c1], n[c0:c1]])
+        assert np.isnan(out[:, c1 - c0:]).all()  # nothing past the block's columns
+    with pytest.raises(RuntimeError):
+        ints.coulomb_3c_pairs(aux, b, 0, npairs, np.empty((aux.size, 4)))

@@ -41,6 +41,21 @@ def test_device_B_matches_oracle_B(dev):
     assert np.abs(Bd - Bo).max() < 1e-12
 
 
+def test_device_B_pair_blocks_match_oracle_B(dev):
+    # many pair blocks (1 MiB each): pipeline boundaries, exact m/n symmetry
+    from forte2_amd.df import build_B_density_fitting
+
+    s = System(geo.water_cluster(4, seed=4), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    st = {}
+    Bd = build_B_density_fitting(s, dev, block_bytes=1 << 20, stats=st)
+    assert st["blocks"] > 10
+    Bo = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    Bh = Bd.cpu().numpy()
+    print(f"   |dB| = {np.abs(Bh - Bo).max():.2e} over {st['blocks']} blocks", flush=True)
+    assert np.abs(Bh - Bo).max() < 1e-11
+    assert np.array_equal(Bh, Bh.transpose(0, 2, 1))
+
+
 def test_rhf_water_plain_form(dev):
     print("[scf-gpu] water cc-pVDZ/cc-pVTZ-JKFIT RHF", flush=True)
     s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
