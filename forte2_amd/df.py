@@ -11,6 +11,7 @@ With ``df_ortho_rtol`` set, X = M^-1/2 with eigenvalues below rtol*max
 discarded (forte2/helpers/matrix_functions.py:52-92), as in the reference.
 """
 
+import numpy as np
 import torch
 
 from . import integrals as _ints
@@ -171,20 +172,37 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
 
 
 def host_df_integrals(system):
-    """(M, T) on the host: the Coulomb metric and all of (Q|mn) (small jobs)."""
-    return (_ints.coulomb_2c(system.auxiliary_basis),
-            _ints.coulomb_3c(system.auxiliary_basis, system.basis))
+    """Host part of B formation for a job prepared off the serving thread:
+    (M, packed) with M = (P|Q) and packed = (T, m, n), T the unique half of
+    (Q|mn) over the significant shell pairs as a (naux, ncols) array and m, n
+    the basis functions of its columns (pair_columns)."""
+    import numpy as np
+
+    aux, bas = system.auxiliary_basis, system.basis
+    col0, m, n = pair_columns(bas)
+    T = np.empty((aux.size, int(col0[-1])))
+    if T.shape[1]:
+        _ints.coulomb_3c_pairs(aux, bas, 0, len(col0) - 1, T)
+    return _ints.coulomb_2c(aux), (T, m, n)
 
 
-def device_B_from_host(M, T, device, df_ortho_rtol=None, solve_bytes=1 << 30):
-    """B = L^-1 T on the device from host (M, T) (same algebra as build_B_density_fitting)."""
-    Q, N = T.shape[0], T.shape[1]
+def device_B_from_host(M, packed, device, nbf, df_ortho_rtol=None, block_bytes=1 << 30):
+    """B = X T on the device from the host_df_integrals output (same algebra
+    as build_B_density_fitting): the packed columns are multiplied by X in
+    blocks and scattered to (m, n) and (n, m)."""
+    T, m, n = packed
+    Q, N = T.shape[0], int(nbf)
     kind, fac = metric_factor(torch.from_numpy(M).to(device), df_ortho_rtol)
-    B = torch.from_numpy(T).to(device)
     X = _fitting_matrix(kind, fac)
+    B = torch.zeros((Q, N, N), dtype=torch.float64, device=device)
     Bv = B.view(Q, N * N)
-    chunk = max(1, min(N * N, solve_bytes // (8 * Q)))
-    for c0 in range(0, N * N, chunk):
-        c1 = min(N * N, c0 + chunk)
-        Bv[:, c0:c1] = X @ Bv[:, c0:c1].contiguous()
+    idx_mn = torch.from_numpy(m * N + n).to(device)
+    idx_nm = torch.from_numpy(n * N + m).to(device)
+    ncols = T.shape[1]
+    width = max(1, block_bytes // (8 * Q))
+    for c0 in range(0, ncols, width):
+        c1 = min(ncols, c0 + width)
+        Bp = X @ torch.from_numpy(np.ascontiguousarray(T[:, c0:c1])).to(device)
+        Bv.index_copy_(1, idx_nm[c0:c1], Bp)
+        Bv.index_copy_(1, idx_mn[c0:c1], Bp)
     return B

@@ -154,7 +154,7 @@ class SCFServer:
     def _activate(self, req, prepared):
         system, M, T, V_sap = prepared
         system.device = self.device
-        B = _df.device_B_from_host(M, T, self.device)
+        B = _df.device_B_from_host(M, T, self.device, system.nbf, getattr(system, "df_ortho_rtol", None))
         fb = FockBuilder(system, B=B, device=self.device)
         # launch shape of the job's size class (tuned for grouped builds when
         # batching); the X chunk is pinned so that the job's builds that run
