@@ -172,42 +172,47 @@ class FockBuilderOTF:
     def _sweep(self):
         """Yields (p0, p1, T) for every aux-shell block, T = (P|mn) on the
         device, (p1 - p0, N, N).  T is valid until the next block is
-        requested (stream-ordered: work queued on it before then is safe)."""
+        requested (stream-ordered: work queued on it before then is safe).
+        However the sweep ends (also when the consumer stops early), the
+        pending host block is waited for and the copies out of the staging
+        buffers are complete before the next sweep writes them again."""
         N = self.nbf
         todo = [k for k in range(len(self.blocks)) if k not in self._cache]
         copied = [None, None]  # event: the device copy out of that staging buffer is done
         fut = self._pool.submit(self._host_block, todo[0], 0) if todo else None
         it = 0
         stream = torch.cuda.current_stream(self.device)
-        for k, (s0, s1, p0, p1) in enumerate(self.blocks):
-            if k in self._cache:
-                yield p0, p1, self._cache[k]
-                continue
-            slot = it % 2
-            fut.result()
-            self.host_blocks_computed += 1
-            if it + 1 < len(todo):
-                nxt = (it + 1) % 2
-                if copied[nxt] is not None:
-                    copied[nxt].synchronize()
-                fut = self._pool.submit(self._host_block, todo[it + 1], nxt)
-            nP = p1 - p0
-            nbytes = nP * N * N * 8
-            if self.cache_bytes - sum(t.numel() * 8 for t in self._cache.values()) >= nbytes:
-                T = torch.empty((nP, N, N), dtype=torch.float64, device=self.device)
-                self._cache[k] = T
-            else:
-                T = self._dev_block[:nP]
-            T.copy_(self._host[slot][:nP], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(stream)
-            copied[slot] = ev
-            it += 1
-            yield p0, p1, T
-        # the staging buffers are reused by the next sweep's host work
-        for ev in copied:
-            if ev is not None:
-                ev.synchronize()
+        try:
+            for k, (s0, s1, p0, p1) in enumerate(self.blocks):
+                if k in self._cache:
+                    yield p0, p1, self._cache[k]
+                    continue
+                slot = it % 2
+                fut.result()
+                self.host_blocks_computed += 1
+                if it + 1 < len(todo):
+                    nxt = (it + 1) % 2
+                    if copied[nxt] is not None:
+                        copied[nxt].synchronize()
+                    fut = self._pool.submit(self._host_block, todo[it + 1], nxt)
+                nP = p1 - p0
+                if self.cache_bytes - sum(t.numel() * 8 for t in self._cache.values()) >= nP * N * N * 8:
+                    T = torch.empty((nP, N, N), dtype=torch.float64, device=self.device)
+                    self._cache[k] = T
+                else:
+                    T = self._dev_block[:nP]
+                T.copy_(self._host[slot][:nP], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(stream)
+                copied[slot] = ev
+                it += 1
+                yield p0, p1, T
+        finally:
+            if fut is not None:
+                fut.result()
+            for ev in copied:
+                if ev is not None:
+                    ev.synchronize()
 
     def _half_transform(self, T, C, out):
         """out[P, m, i] = sum_n T_Pmn C_ni on the device (MFMA kernel)."""
