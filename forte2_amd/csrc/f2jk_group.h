@@ -100,7 +100,7 @@ __device__ __forceinline__ void mirror_reduce(const double* part, int nsplit, in
 // PREC: the group's precision; NBW, VA: tile width and load width of the
 // BF16x3 half-transform (unused otherwise).
 template <int PHASE, int PREC, int NBW = 2, int VA = 2>
-__global__ __launch_bounds__(256, PREC != F2JK_PREC_BF16X3 ? 1
+__global__ __launch_bounds__(256, PREC != F2JK_PREC_BF16X3 ? (PHASE == GP_KX || PHASE == GP_KS ? 2 : 1)
                                   : PHASE == GP_KX ? kx_bf16_waves_per_eu<NBW>()
                                   : PHASE == GP_KS ? 2 : 1)
 void grouped_kernel(const GArgs a) {

@@ -654,8 +654,10 @@ __device__ __forceinline__ void ks_kernel_body(const T* __restrict__ X, int N, i
             }
         }
 }
+// two workgroups per CU: 4 waves of one workgroup leave each SIMD idle at the
+// k-step barrier and the LDS-read latency
 template <typename T>
-__global__ __launch_bounds__(kGThreads) void ks_kernel(const T* __restrict__ X, int N, int OPAD,
+__global__ __launch_bounds__(kGThreads, 2) void ks_kernel(const T* __restrict__ X, int N, int OPAD,
                                                        int ioff, int width, int Pc, int nsplit,
                                                        int nlower, double* __restrict__ Kpart,
                                                        int accumulate) {
