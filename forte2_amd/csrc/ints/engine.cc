@@ -646,6 +646,40 @@ int64_t pair_ncols(const f2ints_basis* bs, const std::pair<int, int>& ab) {
     return ab.first == ab.second ? (int64_t)na * (na + 1) / 2 : (int64_t)na * nb;
 }
 
+// V[m][n] = sum over every aux function P of (P|mn), both triangles
+int coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out) {
+    const int64_t nbf = bs->nbf;
+    const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
+    std::fill(out, out + (size_t)nbf * nbf, 0.0);
+    int maxlaux = 0;
+    for (const Shell& P : aux->shells) maxlaux = std::max(maxlaux, P.l);
+#pragma omp parallel num_threads(nthreads())
+    {
+        Scratch3c w(maxlaux, bs->maxl);
+        std::vector<double> sum;
+#pragma omp for schedule(dynamic)
+        for (size_t q = 0; q < pairs.size(); ++q) {
+            const Shell& A = bs->shells[pairs[q].first];
+            const Shell& B = bs->shells[pairs[q].second];
+            const int nsa = nsph(A.l), nsbb = nsph(B.l), nab = nsa * nsbb;
+            sum.assign(nab, 0.0);
+            for (const Shell& P : aux->shells) {
+                eval_3c(bs, q, P, w);
+                for (int sP = 0; sP < nsph(P.l); ++sP)
+                    for (int k = 0; k < nab; ++k) sum[k] += w.acc[(size_t)sP * nab + k];
+            }
+            // pair q owns these entries: no other thread writes them
+            for (int sa = 0; sa < nsa; ++sa)
+                for (int sb = 0; sb < nsbb; ++sb) {
+                    const int64_t m = A.off + sa, n = B.off + sb;
+                    out[m * nbf + n] = sum[sa * nsbb + sb];
+                    out[n * nbf + m] = sum[sa * nsbb + sb];
+                }
+        }
+    }
+    return 0;
+}
+
 // (P|ab) for every aux function and the significant pairs [q0, q1), packed:
 // out[P * ldo + c], column c running over the pairs in order; a pair a > b
 // has nsa * nsb columns (row-major in (sa, sb)), a pair a == b the
@@ -851,6 +885,11 @@ int64_t f2ints_ket_pairs(const f2ints_basis* bs, int* a, int* b, int64_t cap) {
             b[q] = pairs[q].second;
         }
     return n;
+}
+
+int f2ints_coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out) {
+    if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_sum: null argument");
+    return coulomb_3c_sum(aux, bs, out);
 }
 
 int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1,

@@ -100,6 +100,14 @@ def coulomb_3c_pairs(aux, basis, q0, q1, out):
     return out
 
 
+def coulomb_3c_sum(aux, basis):
+    """V[m, n] = sum_P (P|mn) over every function of `aux` (nbf x nbf)."""
+    n = basis.size
+    V = np.empty((n, n))
+    _check(_lib().f2ints_coulomb_3c_sum(aux.handle, basis.handle, _ptr(V)))
+    return V
+
+
 def coulomb_4c(basis):
     n = basis.size
     out = np.empty((n, n, n, n))

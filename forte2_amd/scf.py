@@ -26,6 +26,7 @@ reference build, the error of the low-precision energy against FP64.
 
 import time
 
+import numpy as np
 import torch
 
 
@@ -51,6 +52,7 @@ class DeviceDIIS:
 
     def reset(self):
         self.p, self.e = [], []
+        self.G = None  # Gram matrix <e_i, e_j> of the stored error vectors (host FP64)
         self.iter = 0
         self.status = ""
 
@@ -61,27 +63,47 @@ class DeviceDIIS:
         self.iter += 1
         if self.iter < self.start:
             return p
+        e = e.reshape(-1).clone()
+        # one new row of the Gram matrix: dot products of the new error vector
+        # with the stored ones (memory-bound reductions, one sync)
+        row = torch.stack([torch.dot(e, ei) for ei in self.e] + [torch.dot(e, e)]).cpu().numpy()
+        n_old = len(self.e)
+        G = np.zeros((n_old + 1, n_old + 1))
+        if n_old:
+            G[:n_old, :n_old] = self.G
+        G[n_old, :] = row
+        G[:, n_old] = row
         self.p.append(p.clone())
-        self.e.append(e.reshape(-1).clone())
+        self.e.append(e)
         if len(self.p) > self.nvec:
             self.p.pop(0)
             self.e.pop(0)
+            G = G[1:, 1:]
+        self.G = G
         self.status = "S"
         n = len(self.p)
         if n < self.min:
             return p
-        E = torch.stack(self.e)
-        Bm = -torch.ones((n + 1, n + 1), dtype=p.dtype, device=p.device)
+        # the (n+1) x (n+1) CDIIS system of diis.py:104-117, normalised by max|B|
+        Bm = -np.ones((n + 1, n + 1))
         Bm[-1, -1] = 0.0
-        G = E @ E.T
-        Bm[:n, :n] = G / G.abs().max()
-        rhs = torch.zeros(n + 1, dtype=p.dtype, device=p.device)
+        with np.errstate(invalid="ignore", divide="ignore"):  # all-zero errors: caught below
+            Bm[:n, :n] = G / np.abs(G).max()
+        rhs = np.zeros(n + 1)
         rhs[-1] = -1.0
-        x, info = torch.linalg.solve_ex(Bm, rhs)
-        if int(info) != 0 or not bool(torch.isfinite(x).all()):
+        try:
+            if not np.isfinite(Bm).all():
+                raise np.linalg.LinAlgError
+            x = np.linalg.solve(Bm, rhs)
+        except np.linalg.LinAlgError:
             return p  # singular DIIS system: skip extrapolation (diis.py:115-120)
+        if not np.isfinite(x).all():
+            return p
         self.status += "/E"
-        return torch.einsum("i,i...->...", x[:n], torch.stack(self.p))
+        out = self.p[0] * float(x[0])
+        for xi, pi in zip(x[1:n], self.p[1:]):
+            out.add_(pi, alpha=float(xi))
+        return out
 
 
 def _procrustes_split(Cn, Co, S):

@@ -45,6 +45,7 @@ class System:
         self.nuclear_repulsion = _ints.nuclear_repulsion(self.atoms)
         self._S = None
         self._H = None
+        self._V_sap = None
         self.Xorth, self.nmo = canonical_orth(self.ints_overlap(), overlap_ortho_rtol)
         self.two_component = False
         self.device = device
@@ -83,13 +84,12 @@ class System:
         return _ints.coulomb_3c(self.auxiliary_basis, self.basis, shell_begin, shell_end, out)
 
     def sap_potential(self):
-        """V_SAP = sum_P (P|mn) over the SAP charge-density basis (scf_utils.py:10-51)."""
-        dens = _basis.build_sap_density_basis(self.atoms)
-        V = np.zeros((self.nbf, self.nbf))
-        step = 64
-        for s0 in range(0, dens.nshells, step):
-            V += _ints.coulomb_3c(dens, self.basis, s0, min(dens.nshells, s0 + step)).sum(axis=0)
-        return V
+        """V_SAP = sum_P (P|mn) over the SAP charge-density basis (scf_utils.py:10-51),
+        summed over P inside the engine; computed once per System (it depends
+        on the geometry and basis only)."""
+        if self._V_sap is None:
+            self._V_sap = _ints.coulomb_3c_sum(_basis.build_sap_density_basis(self.atoms), self.basis)
+        return self._V_sap.copy()
 
     # ------------------------------------------------------------------ Fock builder slot
     @property

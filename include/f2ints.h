@@ -75,6 +75,12 @@ int64_t f2ints_ket_pairs(const f2ints_basis* bs, int* a, int* b, int64_t cap);
 int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t pair_begin,
                             int64_t pair_end, double* out, int64_t ldo);
 
+/* V[m][n] = sum over all functions P of `aux` of (P|mn), nbf x nbf: the
+ * superposition-of-atomic-potentials guess potential of
+ * forte2/scf/scf_utils.py:10-51 (aux = the SAP charge-density basis), without
+ * forming (P|mn) per P. */
+int f2ints_coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out);
+
 /* (mn|rs) chemist's notation, nbf^4 -- small bases only (tests). */
 int f2ints_coulomb_4c(const f2ints_basis* b, double* out);
 

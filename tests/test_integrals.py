@@ -170,3 +170,14 @@ def test_three_center_pair_blocks_match_square():
         assert np.isnan(out[:, c1 - c0:]).all()  # nothing past the block's columns
     with pytest.raises(RuntimeError):
         ints.coulomb_3c_pairs(aux, b, 0, npairs, np.empty((aux.size, 4)))
+
+
+def test_three_center_sum_matches_blocks():
+    # f2ints_coulomb_3c_sum = sum over P of the (P|mn) blocks (the SAP guess potential)
+    atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5")
+    b = bs.build_basis("cc-pvdz", atoms)
+    dens = bs.build_sap_density_basis(atoms)
+    ref = ints.coulomb_3c(dens, b).sum(axis=0)
+    V = ints.coulomb_3c_sum(dens, b)
+    assert np.abs(V - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.array_equal(V, V.T)
