@@ -145,3 +145,20 @@ def test_error_paths(dev):
         fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)] * 9)
     with pytest.raises(ValueError):
         fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)], precision="fp16")
+
+
+def test_edge_shapes(dev):
+    # an empty density (UHF with no electron of one spin), nocc = nbf, and a
+    # single auxiliary function
+    for Q, N, noccs in ((60, 40, [0, 7]), (60, 40, [40]), (1, 33, [5])):
+        B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=Q + N + len(noccs))
+        if noccs[0] == 0:
+            Cs[0] = torch.zeros((N, 0), dtype=torch.float64, device=dev)
+            Ch[0] = np.zeros((N, 0))
+        J, K = FockBuilder(B=B, device=dev).build_JK(Cs)
+        Jr, Kr = ojk.build_JK(Bh, Ch)
+        for k in range(len(noccs)):
+            assert np.abs(J[k].cpu().numpy() - Jr[k]).max() <= TOL_FP64
+            assert np.abs(K[k].cpu().numpy() - Kr[k]).max() <= TOL_FP64
+        if noccs[0] == 0:
+            assert float(J[0].abs().max()) == 0.0 and float(K[0].abs().max()) == 0.0

@@ -104,3 +104,12 @@ def test_mp2_golden():
     den = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
     emp2 = r.E + np.sum(V * (2.0 * V - V.transpose(0, 1, 3, 2)) / den)
     assert emp2 == pytest.approx(-76.3710978841482984, abs=5e-8)
+
+
+def test_uhf_one_electron_golden():
+    # tests/scf/test_uhf.py:44-55: H2+ with ms = -1/2 (no alpha electron)
+    s = System("H 0 0 0\nH 0 0 1.0", "cc-pvqz", "cc-pvqz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    r = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - 1, -0.5,
+                     X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+    assert r.E == pytest.approx(-0.601864064744, abs=5e-8)

@@ -142,3 +142,13 @@ def test_metric_inverse_sqrt_branch_gives_same_jk(dev):
     Js, Ks = FockBuilder(B=Bs, device=dev).build_JK([C])
     assert float((Jc[0] - Js[0]).abs().max()) < 1e-10
     assert float((Kc[0] - Ks[0]).abs().max()) < 1e-10
+
+
+def test_uhf_one_electron_golden(dev):
+    # tests/scf/test_uhf.py:44-55: H2+ with ms = -1/2, so the alpha density is
+    # empty (a zero-column C in every build)
+    s = System("H 0 0 0\nH 0 0 1.0", "cc-pvqz", "cc-pvqz-jkfit", device=dev)
+    scf = UHF(charge=1, ms=-0.5, e_tol=1e-10)(s).run()
+    print(f"   E = {scf.E:.12f} <S^2> = {scf.S2:.6f}", flush=True)
+    assert scf.E == pytest.approx(-0.601864064744, abs=5e-8)
+    assert scf.S2 == pytest.approx(0.75, abs=5e-8)
