@@ -62,6 +62,7 @@ struct f2jk_ctx {
     // J (HBM-bound) runs on a side stream concurrently with K (MFMA-bound)
     bool overlap_jk = false;
     int kx_waves = 8;               // waves per BF16x3 half-transform workgroup (4 or 8)
+    int ks_tile = 0;                // BF16x3 contraction tile edge: 128, 256, 0 = by nbf
     hipStream_t side = nullptr;     // default priority, created with the context
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int target_blocks = kTargetBlocks;
@@ -92,6 +93,7 @@ struct Plan {
     size_t jb = 8;             // bytes of a B element read by J
     // K
     int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0, kx_waves = 4;
+    int ks_tile = 128;  // edge of the contraction's lower tiles (256: BF16x3 8-wave tiles)
     // workspace offsets
     int KP = 0;                // nbf rounded up to the bf16 k step
     size_t off_cp32 = 0, off_cth = 0, off_ctl = 0;
@@ -123,7 +125,7 @@ int ensure_rowoff(f2jk_ctx* ctx, int vec) {
 }
 
 int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
-              bool j_from_c, Plan& p, int jvec_override = 0) {
+              bool j_from_c, Plan& p, int jvec_override = 0, bool grouped = false) {
     if (ctx->nbf <= 0 || ctx->naux <= 0) return fail(F2JK_ESTATE, "B has not been set");
     if (ndens < 1 || ndens > kMaxDens)
         return fail(F2JK_EINVAL, "ndens must be in [1, " + std::to_string(kMaxDens) + "]");
@@ -200,9 +202,15 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         pc = std::max<int64_t>(1, std::min<int64_t>(pc, Q));
         p.Pc = (int)pc;
         p.nchunks = (int)cdiv(Q, pc);
-        p.ntile = (int)cdiv(N, kTile);
+        // BF16x3 single-job builds of large nbf contract in 256-wide tiles
+        // (one 8-wave workgroup per CU); everything else in 128-wide tiles
+        p.ks_tile = 128;
+        if (precision == F2JK_PREC_BF16X3 && !grouped)
+            p.ks_tile = ctx->ks_tile ? ctx->ks_tile : (N >= 768 ? 256 : 128);
+        p.ntile = (int)cdiv(N, p.ks_tile);
         p.nlower = p.ntile * (p.ntile + 1) / 2;
-        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(ctx->target_blocks / 2, p.nlower)));
+        const int64_t wg_target = p.ks_tile == 256 ? ctx->target_blocks / 4 : ctx->target_blocks / 2;
+        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(wg_target, p.nlower)));
         if (precision == F2JK_PREC_BF16X3) {
             // the tile width (in 32-column units) within the cap that computes
             // the fewest column blocks; ties go to the wider tile
@@ -479,9 +487,14 @@ int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipSt
         TIM(F2JK_PH_KS, true);
         const int nsplit = std::min(p.nsplit, pc);
         for (int d = 0; d < p.nd; ++d) {
-            hipLaunchKernelGGL(ks_bf16x3_kernel, dim3(p.nlower * nsplit), dim3(kGThreads), 0, st, xh, xl,
-                               p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
-                               kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
+            if (p.ks_tile == 256)
+                hipLaunchKernelGGL((ks_bf16x3_kernel<256, 4, 2>), dim3(p.nlower * nsplit), dim3(512), 0, st,
+                                   xh, xl, p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
+                                   kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
+            else
+                hipLaunchKernelGGL((ks_bf16x3_kernel<kTile, 2, 2>), dim3(p.nlower * nsplit), dim3(kGThreads), 0,
+                                   st, xh, xl, p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
+                                   kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
             F2JK_LAUNCHED("ks_bf16x3");
         }
         TIM(F2JK_PH_KS, false);
@@ -580,7 +593,7 @@ int group_plan(f2jk_ctx* const* ctxs, int njobs, const int* ndens, const int64_t
         if (!ctxs[j]) return fail(F2JK_EINVAL, "null context in group");
         if (ndens[j] < 1 || ndens[j] > 2) return fail(F2JK_EUNSUPPORTED, "grouped jobs have 1 or 2 densities");
         // one J vector width (2) for the whole group, so one kernel serves every job
-        int rc = make_plan(ctxs[j], ndens[j], nocc + k, precision, flags, true, plans[j], 2);
+        int rc = make_plan(ctxs[j], ndens[j], nocc + k, precision, flags, true, plans[j], 2, true);
         if (rc) return rc;
         const Plan& p = plans[j];
         if (p.N % 2) return fail(F2JK_EUNSUPPORTED, "grouped builds need an even nbf");
@@ -748,6 +761,9 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
         ctx->bf16_nbw_max = (int)value;
     } else if (!strcmp(key, "overlap_jk")) {
         ctx->overlap_jk = value != 0;
+    } else if (!strcmp(key, "ks_tile")) {
+        if (value != 0 && value != 128 && value != 256) return fail(F2JK_EINVAL, "ks_tile must be 0, 128 or 256");
+        ctx->ks_tile = (int)value;
     } else if (!strcmp(key, "kx_waves")) {
         if (value != 4 && value != 8) return fail(F2JK_EINVAL, "kx_waves must be 4 or 8");
         ctx->kx_waves = (int)value;

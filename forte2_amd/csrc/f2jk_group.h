@@ -217,7 +217,7 @@ void grouped_kernel(const GArgs a) {
             double* kp = g.Kpart + (long long)d * g.nsplit * g.N * g.N;
             if constexpr (PREC == F2JK_PREC_BF16X3) {
                 const u16* xh = (const u16*)g.X;
-                ks_bf16x3_kernel_body(xh, xh + (long long)g.Q * g.N * g.OPAD, g.N, g.OPAD, g.ioff[d], g.width[d],
+                ks_bf16x3_kernel_body<kTile, 2, 2>(xh, xh + (long long)g.Q * g.N * g.OPAD, g.N, g.OPAD, g.ioff[d], g.width[d],
                                       g.Q, g.nsplit, g.nlower, kp, 0, b, per);
             } else {
                 ks_kernel_body<TB>((const TB*)g.X, g.N, g.OPAD, g.ioff[d], g.width[d], g.Q, g.nsplit, g.nlower,

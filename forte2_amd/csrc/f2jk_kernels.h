@@ -943,15 +943,24 @@ __global__ __launch_bounds__(64 * WAVES, (kx_bf16_waves_per_eu<NBW, WAVES>())) v
 // KS (bf16x3): Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
 // with X given as hi/lo bf16 pairs; width % 32 == 0.  FP32 accumulation over
 // the P range of a split (the MFMA rounds once per 32 products), FP64 across
-// splits.  With no FP64 registers the kernel fits two workgroups per CU; the
-// split length (target_blocks) sets the length of the FP32 sums.
+// splits.  Lower TS x TS tiles; WR x WC waves, each holding a (TS/WR) x
+// (TS/WC) block of the tile.  TS = 128 with 2 x 2 waves fits two workgroups
+// per CU (no FP64 registers); TS = 256 with 4 x 2 waves is one 8-wave
+// workgroup per CU that reads half the LDS bytes per MFMA (each fragment
+// feeds more blocks), for large nbf.  The split length (target_blocks) sets
+// the length of the FP32 sums.
+template <int TS, int WR, int WC>
 __device__ __forceinline__ void ks_bf16x3_kernel_body(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
     int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate, int b, int nblocks) {
-    __shared__ __bf16 Ah[2][kTile][kBK16];
-    __shared__ __bf16 Al[2][kTile][kBK16];
-    __shared__ __bf16 Bh[2][kTile][kBK16];
-    __shared__ __bf16 Bl[2][kTile][kBK16];
+    constexpr int T = 64 * WR * WC;        // threads
+    constexpr int IB = TS / WR / 16;       // 16-row blocks per wave
+    constexpr int JB = TS / WC / 16;       // 16-column blocks per wave
+    static_assert(JB >= 2, "staging is spread over the first two column blocks");
+    __shared__ __bf16 Ah[2][TS][kBK16];
+    __shared__ __bf16 Al[2][TS][kBK16];
+    __shared__ __bf16 Bh[2][TS][kBK16];
+    __shared__ __bf16 Bl[2][TS][kBK16];
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -961,12 +970,13 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
     int mt = 0;
     while ((mt + 1) * (mt + 2) / 2 <= tl) ++mt;
     const int nt = tl - mt * (mt + 1) / 2;
-    const int m0 = mt * kTile, n0 = nt * kTile;
-    const int wr = wid >> 1, wc = wid & 1;
+    const int m0 = mt * TS, n0 = nt * TS;
+    const int wr = wid / WC, wc = wid % WC;
     const int Pa = (int)(((int64_t)Pc * s) / nsplit);
     const int Pb = (int)(((int64_t)Pc * (s + 1)) / nsplit);
 
-    constexpr int LD = (kTile * kBK16 / 8) / kGThreads;  // 16-byte loads per array (= 2)
+    constexpr int LD = (TS * kBK16 / 8) / T;  // 16-byte loads per array (= 2)
+    static_assert(LD * T * 8 == TS * kBK16, "tile rows must split evenly over the threads");
     u32x4 rah[LD], ral[LD], rbh[LD], rbl[LD];
 
     // Branch-free loads: rows past N are clamped onto row N-1 and feed only
@@ -975,7 +985,7 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         const int64_t base = (int64_t)P * N * OPAD + ioff + i0;
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
-            const int idx = tid + kGThreads * q;
+            const int idx = tid + T * q;
             const int row = idx >> 2, kc = (idx & 3) * 8;
             const int64_t o = base + (int64_t)min(m0 + row, N - 1) * OPAD + kc;
             rah[q] = *reinterpret_cast<const u32x4*>(Xh + o);
@@ -986,7 +996,7 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         const int64_t base = (int64_t)P * N * OPAD + ioff + i0;
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
-            const int idx = tid + kGThreads * q;
+            const int idx = tid + T * q;
             const int row = idx >> 2, kc = (idx & 3) * 8;
             const int64_t o = base + (int64_t)min(n0 + row, N - 1) * OPAD + kc;
             rbh[q] = *reinterpret_cast<const u32x4*>(Xh + o);
@@ -996,7 +1006,7 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
     auto store_a = [&](int stg) {
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
-            const int idx = tid + kGThreads * q;
+            const int idx = tid + T * q;
             const int row = idx >> 2, e = swz16(row, idx & 3);
             *reinterpret_cast<u32x4*>(&Ah[stg][row][e]) = rah[q];
             *reinterpret_cast<u32x4*>(&Al[stg][row][e]) = ral[q];
@@ -1005,18 +1015,18 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
     auto store_b = [&](int stg) {
 #pragma unroll
         for (int q = 0; q < LD; ++q) {
-            const int idx = tid + kGThreads * q;
+            const int idx = tid + T * q;
             const int row = idx >> 2, e = swz16(row, idx & 3);
             *reinterpret_cast<u32x4*>(&Bh[stg][row][e]) = rbh[q];
             *reinterpret_cast<u32x4*>(&Bl[stg][row][e]) = rbl[q];
         }
     };
 
-    fx4 acc[4][4];
+    fx4 acc[IB][JB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fx4{0, 0, 0, 0};
+        for (int j = 0; j < JB; ++j) acc[i][j] = fx4{0, 0, 0, 0};
 
     const int nkp = width / kBK16;
     // steps are numbered across P; step st+1's tiles are written to the other
@@ -1040,6 +1050,7 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
     }
     __syncthreads();
     const int fr = lane & 15, fc = lane >> 4;  // fragment row / 16-byte chunk of this lane
+    const int ra0 = wr * (TS / WR), rb0w = wc * (TS / WC);
     int st = 0;
     for (int P = Pa; P < Pb; ++P) {
     for (int kt = 0; kt < nkp; ++kt, ++st) {
@@ -1047,33 +1058,33 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
         const int sp = min(st + 2, nsteps - 1);  // clamped prefetch: the last loads are harmless repeats
         const int Ppre = Pa + sp / nkp, ipre = (sp % nkp) * kBK16;
         // fragment reads in the order the first group consumes them
-        bf16x8 ah[4], al[4], bh[2], bl[2];
-        const int rb0 = wc * 64 + fr;
+        bf16x8 ah[IB], al[IB], bh[2], bl[2];
+        const int rb0 = rb0w + fr;
         bh[0] = *reinterpret_cast<const bf16x8*>(&Bh[cur][rb0][swz16(rb0, fc)]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = wr * 64 + i * 16 + fr;
+        for (int i = 0; i < IB; ++i) {
+            const int r = ra0 + i * 16 + fr;
             al[i] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
         }
         bl[0] = *reinterpret_cast<const bf16x8*>(&Bl[cur][rb0][swz16(rb0, fc)]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = wr * 64 + i * 16 + fr;
+        for (int i = 0; i < IB; ++i) {
+            const int r = ra0 + i * 16 + fr;
             ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
         }
         // one group per 16-column block j (as in the half-transform): block
         // j+1's fragments are read and one staging task is issued under the
-        // 12 MFMAs of block j
-        static_for<0, 4>([&](auto J) {
+        // 3 * IB MFMAs of block j
+        static_for<0, JB>([&](auto J) {
             constexpr int j = decltype(J)::value;
-            if constexpr (j + 1 < 4) {
-                const int r = wc * 64 + (j + 1) * 16 + fr;
+            if constexpr (j + 1 < JB) {
+                const int r = rb0w + (j + 1) * 16 + fr;
                 bh[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bh[cur][r][swz16(r, fc)]);
                 bl[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Bl[cur][r][swz16(r, fc)]);
             }
-            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(al[I], bh[j & 1], acc[I][j]); });
-            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bl[j & 1], acc[I][j]); });
-            static_for<0, 4>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bh[j & 1], acc[I][j]); });
+            static_for<0, IB>([&](auto I) { acc[I][j] = mfma_bf16(al[I], bh[j & 1], acc[I][j]); });
+            static_for<0, IB>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bl[j & 1], acc[I][j]); });
+            static_for<0, IB>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bh[j & 1], acc[I][j]); });
             if constexpr (j == 0) {
                 store_a(nxt);
                 load_a(Ppre, ipre);
@@ -1089,14 +1100,14 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
     }
     double* Ks = Kpart + (int64_t)s * N * N;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+        for (int j = 0; j < JB; ++j) {
+            const int col = n0 + rb0w + j * 16 + (lane & 15);
             if (col >= N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int row = m0 + ra0 + i * 16 + 4 * (lane >> 4) + r;
                 if (row >= N) continue;
                 double* dst = Ks + (int64_t)row * N + col;
                 const double v = (double)acc[i][j][r];
@@ -1104,11 +1115,12 @@ __device__ __forceinline__ void ks_bf16x3_kernel_body(
             }
         }
 }
-__global__ __launch_bounds__(kGThreads, 2) void ks_bf16x3_kernel(
+template <int TS, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC, (TS == 128 ? 2 : 1)) void ks_bf16x3_kernel(
     const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
     int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate) {
-    ks_bf16x3_kernel_body(Xh, Xl, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate, blockIdx.x,
-                          gridDim.x);
+    ks_bf16x3_kernel_body<TS, WR, WC>(Xh, Xl, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate,
+                                      blockIdx.x, gridDim.x);
 }
 
 

@@ -97,7 +97,10 @@ int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
  * half-transform tile at 32 * value occupied columns (tiles of up to 5 run two
  * workgroups per CU); "kx_waves" (4 or 8, default 8) is the number of waves of
  * a BF16x3 half-transform workgroup (8: two per SIMD sharing the tile; tiles
- * of an even number of 32-column units).  Set by the autotuner. */
+ * of an even number of 32-column units); "ks_tile" (0, 128, 256; default 0 =
+ * 256 when nbf >= 768) is the lower-tile edge of the single-job BF16x3
+ * contraction (256: one 8-wave workgroup per CU; grouped builds use 128).
+ * Set by the autotuner. */
 int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value);
 
 /* Scratch bytes needed by f2jk_build_jk / _j / _k for these arguments.

@@ -121,19 +121,25 @@ def test_fp32_precision_form(dev):
         (500, 256, [330]),        # more than 320 occupied columns: two column tiles
     ],
 )
-def test_bf16x3_precision_form(dev, Q, N, noccs):
+@pytest.mark.parametrize("ks_tile", [128, 256])
+def test_bf16x3_precision_form(dev, Q, N, noccs, ks_tile):
+    # ks_tile: the contraction's lower-tile edge (256: one 8-wave workgroup
+    # per CU, chosen by default for nbf >= 768; forced here at every size,
+    # including a single partial tile and odd nbf)
     B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=17)
     fb = FockBuilder(B=B, device=dev)
+    fb.set_tuning("ks_tile", ks_tile)
     J, K = fb.build_JK(Cs, precision="bf16x3")
     Jr, Kr = ojk.build_JK(Bh, Ch)
     for k in range(len(noccs)):
         rj = np.abs(J[k].cpu().numpy() - Jr[k]).max() / np.abs(Jr[k]).max()
         rk = np.abs(K[k].cpu().numpy() - Kr[k]).max() / np.abs(Kr[k]).max()
-        print(f"   bf16x3 dens {k}: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
+        print(f"   bf16x3 tile {ks_tile} dens {k}: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
         # J streams the FP32 copy of B (FP64 accumulation); K carries ~16
         # significant bits per operand (hi/lo bf16 pair) with FP32 accumulation
         assert rj < 1e-6
         assert rk < 1e-4
+        assert torch.equal(K[k], K[k].T)
 
 
 def test_error_paths(dev):
