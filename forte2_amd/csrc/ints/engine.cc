@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "f2ints.h"
@@ -257,11 +258,15 @@ struct f2ints_basis {
     std::vector<Shell> shells;
     int64_t nbf = 0;
     int maxl = 0;
-    // Hermite expansions of all significant shell pairs (ket form), built once
-    // on the first 3-centre call and reused by every later aux-shell block
+    // Significant shell pairs (a >= b) and the ket-form Hermite data of their
+    // shell families (consecutive shells of one centre, l and exponent set:
+    // the columns of a general contraction), built once on the first
+    // 3-centre call and reused by every later block
     std::once_flag pairs_once;
     std::vector<std::pair<int, int>> pair_index;
-    std::vector<void*> pair_data;  // ShellPair*, owned
+    std::vector<void*> fam_pairs;   // FamPair*, owned
+    std::vector<int> fam_of_pair;   // family pair of each significant shell pair
+    std::vector<int> col_of_pair;   // its column pair within that family pair
     ~f2ints_basis();
 };
 
@@ -309,15 +314,18 @@ struct ShellPair {
     std::vector<PrimPair> prims;
 };
 
+// Spherical Hermite expansion E[sa * nsb + sb][h] of the primitive pair
+// (a on A, b on B) without contraction coefficients; ket form flips odd h.
+struct PrimE {
+    E1D ex, ey, ez;
+    std::vector<double> ecart, etmp;
+    void compute(const Shell& A, const Shell& B, double a, double b, bool ket, std::vector<double>& E);
+};
+
 void make_pair(const Shell& A, const Shell& B, bool ket, ShellPair& sp) {
-    const Harmonics& Hm = harm();
-    const HermIndex& H = hidx();
-    const int la = A.l, lb = B.l, L = la + lb, nh = nherm(L);
-    const int nca = ncart(la), ncb = ncart(lb), nsa = nsph(la), nsb = nsph(lb);
     const double AB2 = (A.A[0] - B.A[0]) * (A.A[0] - B.A[0]) + (A.A[1] - B.A[1]) * (A.A[1] - B.A[1]) +
                        (A.A[2] - B.A[2]) * (A.A[2] - B.A[2]);
-    E1D ex, ey, ez;
-    std::vector<double> ecart((size_t)nca * ncb * nh), etmp((size_t)nsa * ncb * nh);
+    PrimE pe;
     sp.prims.clear();
     for (size_t ia = 0; ia < A.alpha.size(); ++ia)
         for (size_t ib = 0; ib < B.alpha.size(); ++ib) {
@@ -329,46 +337,56 @@ void make_pair(const Shell& A, const Shell& B, bool ket, ShellPair& sp) {
             pp.p = p;
             for (int d = 0; d < 3; ++d) pp.P[d] = (a * A.A[d] + b * B.A[d]) / p;
             pp.scale = cc;
-            ex.compute(la, lb, a, b, A.A[0], B.A[0]);
-            ey.compute(la, lb, a, b, A.A[1], B.A[1]);
-            ez.compute(la, lb, a, b, A.A[2], B.A[2]);
-            // Cartesian products
-            for (int ca = 0; ca < nca; ++ca)
-                for (int cb = 0; cb < ncb; ++cb) {
-                    const CartComp& qa = Hm.cart[la][ca];
-                    const CartComp& qb = Hm.cart[lb][cb];
-                    double* dst = &ecart[((size_t)ca * ncb + cb) * nh];
-                    for (int h = 0; h < nh; ++h) {
-                        const auto& tuv = H.list[L][h];
-                        if (tuv[0] > qa.x + qb.x || tuv[1] > qa.y + qb.y || tuv[2] > qa.z + qb.z) {
-                            dst[h] = 0.0;
-                            continue;
-                        }
-                        double v = ex.e[qa.x][qb.x][tuv[0]] * ey.e[qa.y][qb.y][tuv[1]] *
-                                   ez.e[qa.z][qb.z][tuv[2]];
-                        if (ket && ((tuv[0] + tuv[1] + tuv[2]) & 1)) v = -v;
-                        dst[h] = v;
-                    }
-                }
-            // transform a, then b
-            std::fill(etmp.begin(), etmp.end(), 0.0);
-            for (int sa = 0; sa < nsa; ++sa)
-                for (const SphTerm& ta : Hm.sparse[la][sa])
-                    for (int cb = 0; cb < ncb; ++cb) {
-                        const double* src = &ecart[((size_t)ta.cart * ncb + cb) * nh];
-                        double* dst = &etmp[((size_t)sa * ncb + cb) * nh];
-                        for (int h = 0; h < nh; ++h) dst[h] += ta.c * src[h];
-                    }
-            pp.E.assign((size_t)nsa * nsb * nh, 0.0);
-            for (int sa = 0; sa < nsa; ++sa)
-                for (int sb = 0; sb < nsb; ++sb)
-                    for (const SphTerm& tb : Hm.sparse[lb][sb]) {
-                        const double* src = &etmp[((size_t)sa * ncb + tb.cart) * nh];
-                        double* dst = &pp.E[((size_t)sa * nsb + sb) * nh];
-                        for (int h = 0; h < nh; ++h) dst[h] += tb.c * src[h];
-                    }
+            pe.compute(A, B, a, b, ket, pp.E);
             sp.prims.push_back(std::move(pp));
         }
+}
+
+void PrimE::compute(const Shell& A, const Shell& B, double a, double b, bool ket, std::vector<double>& E) {
+    const Harmonics& Hm = harm();
+    const HermIndex& H = hidx();
+    const int la = A.l, lb = B.l, L = la + lb, nh = nherm(L);
+    const int nca = ncart(la), ncb = ncart(lb), nsa = nsph(la), nsb = nsph(lb);
+    ecart.resize((size_t)nca * ncb * nh);
+    etmp.resize((size_t)nsa * ncb * nh);
+    ex.compute(la, lb, a, b, A.A[0], B.A[0]);
+    ey.compute(la, lb, a, b, A.A[1], B.A[1]);
+    ez.compute(la, lb, a, b, A.A[2], B.A[2]);
+    // Cartesian products
+    for (int ca = 0; ca < nca; ++ca)
+        for (int cb = 0; cb < ncb; ++cb) {
+            const CartComp& qa = Hm.cart[la][ca];
+            const CartComp& qb = Hm.cart[lb][cb];
+            double* dst = &ecart[((size_t)ca * ncb + cb) * nh];
+            for (int h = 0; h < nh; ++h) {
+                const auto& tuv = H.list[L][h];
+                if (tuv[0] > qa.x + qb.x || tuv[1] > qa.y + qb.y || tuv[2] > qa.z + qb.z) {
+                    dst[h] = 0.0;
+                    continue;
+                }
+                double v = ex.e[qa.x][qb.x][tuv[0]] * ey.e[qa.y][qb.y][tuv[1]] *
+                           ez.e[qa.z][qb.z][tuv[2]];
+                if (ket && ((tuv[0] + tuv[1] + tuv[2]) & 1)) v = -v;
+                dst[h] = v;
+            }
+        }
+    // transform a, then b
+    std::fill(etmp.begin(), etmp.end(), 0.0);
+    for (int sa = 0; sa < nsa; ++sa)
+        for (const SphTerm& ta : Hm.sparse[la][sa])
+            for (int cb = 0; cb < ncb; ++cb) {
+                const double* src = &ecart[((size_t)ta.cart * ncb + cb) * nh];
+                double* dst = &etmp[((size_t)sa * ncb + cb) * nh];
+                for (int h = 0; h < nh; ++h) dst[h] += ta.c * src[h];
+            }
+    E.assign((size_t)nsa * nsb * nh, 0.0);
+    for (int sa = 0; sa < nsa; ++sa)
+        for (int sb = 0; sb < nsb; ++sb)
+            for (const SphTerm& tb : Hm.sparse[lb][sb]) {
+                const double* src = &etmp[((size_t)sa * ncb + tb.cart) * nh];
+                double* dst = &E[((size_t)sa * nsb + sb) * nh];
+                for (int h = 0; h < nh; ++h) dst[h] += tb.c * src[h];
+            }
 }
 
 int nthreads() { return g_threads > 0 ? g_threads : omp_get_max_threads(); }
@@ -523,62 +541,149 @@ int coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* out) {
 }
 
 // ---------------------------------------------------------------- 3-centre Coulomb
-// significant shell pairs (a >= b) of `bs` with their ket-form Hermite data
+// A shell family is a run of consecutive shells with one centre, l and
+// exponent set: the coefficient columns of a general contraction, which the
+// basis builder splits into one shell per column.  A family pair shares its
+// primitive pairs (and their Hermite expansions, Boys values and aux
+// contractions) between all its column pairs; each column pair only weights
+// them with its own coefficient products.
+struct FamPrim {
+    double p, P[3];
+    std::vector<double> E;  // [nsa * nsb][nh] ket form, no contraction coefficients
+    std::vector<double> w;  // per column pair: ca_i * cb_j, 0 where that pair screens the primitive out
+};
+
+struct FamPair {
+    int a0, na, b0, nb;                     // shell ranges of the two families
+    std::vector<std::pair<int, int>> cols;  // column pairs (ca, cb): shell a0 + ca >= b0 + cb
+    std::vector<int64_t> q;                 // significant shell-pair index of each column pair, -1 if none
+    std::vector<FamPrim> prims;
+};
+
+// significant shell pairs (a >= b) of `bs`, ordered by a then b, with the
+// ket-form Hermite data of their family pairs
 const std::vector<std::pair<int, int>>& ket_pairs(const f2ints_basis* cbs) {
     f2ints_basis* bs = const_cast<f2ints_basis*>(cbs);
     std::call_once(bs->pairs_once, [bs]() {
         const int ns = (int)bs->shells.size();
-        std::vector<std::pair<int, int>> all;
-        for (int a = 0; a < ns; ++a)
-            for (int b = 0; b <= a; ++b) all.push_back({a, b});
-        std::vector<ShellPair*> built(all.size(), nullptr);
-#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
-        for (size_t q = 0; q < all.size(); ++q) {
-            ShellPair* sp = new ShellPair();
-            make_pair(bs->shells[all[q].first], bs->shells[all[q].second], true, *sp);
-            if (sp->prims.empty()) {
-                delete sp;
-                sp = nullptr;
+        std::vector<int> f0;  // first shell of each family
+        for (int s = 0; s < ns; ++s) {
+            const Shell& A = bs->shells[s];
+            bool same = s > 0;
+            if (same) {
+                const Shell& Z = bs->shells[s - 1];
+                same = Z.l == A.l && Z.A[0] == A.A[0] && Z.A[1] == A.A[1] && Z.A[2] == A.A[2] && Z.alpha == A.alpha;
             }
-            built[q] = sp;
+            if (!same) f0.push_back(s);
         }
-        for (size_t q = 0; q < all.size(); ++q)
-            if (built[q]) {
-                bs->pair_index.push_back(all[q]);
-                bs->pair_data.push_back(built[q]);
+        const int nf = (int)f0.size();
+        f0.push_back(ns);
+        std::vector<std::pair<int, int>> fps;
+        for (int fa = 0; fa < nf; ++fa)
+            for (int fb = 0; fb <= fa; ++fb) fps.push_back({fa, fb});
+        std::vector<FamPair*> built(fps.size(), nullptr);
+#pragma omp parallel num_threads(nthreads())
+        {
+            PrimE pe;
+#pragma omp for schedule(dynamic)
+            for (size_t k = 0; k < fps.size(); ++k) {
+                const int fa = fps[k].first, fb = fps[k].second;
+                FamPair* f = new FamPair();
+                f->a0 = f0[fa];
+                f->na = f0[fa + 1] - f0[fa];
+                f->b0 = f0[fb];
+                f->nb = f0[fb + 1] - f0[fb];
+                for (int ca = 0; ca < f->na; ++ca)
+                    for (int cb = 0; cb < f->nb; ++cb)
+                        if (f->a0 + ca >= f->b0 + cb) f->cols.push_back({ca, cb});
+                const int ncp = (int)f->cols.size();
+                const Shell& A = bs->shells[f->a0];
+                const Shell& B = bs->shells[f->b0];
+                const double AB2 = (A.A[0] - B.A[0]) * (A.A[0] - B.A[0]) + (A.A[1] - B.A[1]) * (A.A[1] - B.A[1]) +
+                                   (A.A[2] - B.A[2]) * (A.A[2] - B.A[2]);
+                std::vector<char> used(ncp, 0);
+                for (size_t ia = 0; ia < A.alpha.size(); ++ia)
+                    for (size_t ib = 0; ib < B.alpha.size(); ++ib) {
+                        const double a = A.alpha[ia], b = B.alpha[ib], p = a + b;
+                        const double K = std::exp(-a * b / p * AB2);
+                        const double G = K * std::pow(PI / p, 1.5);
+                        FamPrim pp;
+                        pp.w.assign(ncp, 0.0);
+                        bool any = false;
+                        for (int c = 0; c < ncp; ++c) {
+                            // the screening test of make_pair, per column pair
+                            const double cc = bs->shells[f->a0 + f->cols[c].first].coef[ia] *
+                                              bs->shells[f->b0 + f->cols[c].second].coef[ib];
+                            if (std::fabs(cc) * G < kPrimScreen) continue;
+                            pp.w[c] = cc;
+                            used[c] = 1;
+                            any = true;
+                        }
+                        if (!any) continue;
+                        pp.p = p;
+                        for (int d = 0; d < 3; ++d) pp.P[d] = (a * A.A[d] + b * B.A[d]) / p;
+                        pe.compute(A, B, a, b, true, pp.E);
+                        f->prims.push_back(std::move(pp));
+                    }
+                f->q.assign(ncp, -1);
+                for (int c = 0; c < ncp; ++c)
+                    if (used[c]) f->q[c] = 0;  // significant; numbered below
+                built[k] = f;
             }
+        }
+        // number the significant shell pairs in (a, b) order
+        std::vector<std::tuple<int, int, int, int>> sig;  // (a, b, family pair, column pair)
+        for (size_t k = 0; k < fps.size(); ++k) {
+            FamPair* f = built[k];
+            for (size_t c = 0; c < f->cols.size(); ++c)
+                if (f->q[c] >= 0)
+                    sig.emplace_back(f->a0 + f->cols[c].first, f->b0 + f->cols[c].second, (int)k, (int)c);
+        }
+        std::sort(sig.begin(), sig.end());
+        for (size_t q = 0; q < sig.size(); ++q) {
+            const auto& t = sig[q];
+            bs->pair_index.push_back({std::get<0>(t), std::get<1>(t)});
+            bs->fam_of_pair.push_back(std::get<2>(t));
+            bs->col_of_pair.push_back(std::get<3>(t));
+            built[std::get<2>(t)]->q[std::get<3>(t)] = (int64_t)q;
+        }
+        for (FamPair* f : built) bs->fam_pairs.push_back(f);
     });
     return bs->pair_index;
 }
 
-// scratch of one thread for eval_3c
+// scratch of one thread for eval_fam
 struct Scratch3c {
-    std::vector<double> R, work, Y, acc;
+    std::vector<double> R, work, Y, V, acc;
     Scratch3c(int maxlaux, int maxl) {
         const int Lmax = maxlaux + 2 * maxl;
         R.resize(nherm(Lmax));
         work.resize((size_t)(Lmax + 1) * nherm(Lmax));
         Y.resize((size_t)nsph(maxlaux) * nherm(2 * maxl));
+        V.resize((size_t)nsph(maxlaux) * nsph(maxl) * nsph(maxl));
     }
 };
 
-// (P|ab) of aux shell P and significant pair q of bs into w.acc[sP][sa * nsb + sb]
-void eval_3c(const f2ints_basis* bs, size_t q, const Shell& P, Scratch3c& w) {
+// (P|ab) of aux shell P for every column pair c of family pair f, into
+// w.acc[(c * nsP + sP) * nab + sa * nsb + sb].  Per column pair the terms are
+// added in the order of a single-shell evaluation (aux primitive, then
+// primitive pair), each as (prefactor * coefficient product) * integral.
+void eval_fam(const f2ints_basis* bs, const FamPair& f, const Shell& P, Scratch3c& w) {
     const Harmonics& Hm = harm();
     const HermIndex& H = hidx();
-    const Shell& A = bs->shells[bs->pair_index[q].first];
-    const Shell& B = bs->shells[bs->pair_index[q].second];
-    const ShellPair& sp = *static_cast<const ShellPair*>(bs->pair_data[q]);
+    const Shell& A = bs->shells[f.a0];
+    const Shell& B = bs->shells[f.b0];
     const int lab = A.l + B.l, nhab = nherm(lab), nab = nsph(A.l) * nsph(B.l);
     const int lP = P.l, nsP = nsph(lP), L = lP + lab;
-    w.acc.assign((size_t)nsP * nab, 0.0);
+    const int ncp = (int)f.cols.size();
+    w.acc.assign((size_t)ncp * nsP * nab, 0.0);
     for (size_t k = 0; k < P.alpha.size(); ++k) {
         const double g = P.alpha[k];
-        for (const PrimPair& pp : sp.prims) {
+        for (const FamPrim& pp : f.prims) {
             const double p = pp.p, al = g * p / (g + p);
             compute_R(L, al, P.A[0] - pp.P[0], P.A[1] - pp.P[1], P.A[2] - pp.P[2], w.R.data(), w.work.data());
-            const double pref = 2.0 * std::pow(PI, 2.5) / (g * p * std::sqrt(g + p)) * P.coef[k] *
-                                std::pow(2.0 * g, -lP) * pp.scale;
+            const double base = 2.0 * std::pow(PI, 2.5) / (g * p * std::sqrt(g + p)) * P.coef[k] *
+                                std::pow(2.0 * g, -lP);
             // Y[sP][h] = sum_c T_{lP}(sP, c) R[c + h]
             for (int sP = 0; sP < nsP; ++sP) {
                 double* y = &w.Y[(size_t)sP * nhab];
@@ -591,49 +696,107 @@ void eval_3c(const f2ints_basis* bs, size_t q, const Shell& P, Scratch3c& w) {
                     }
                 }
             }
+            // V[sP][s2] = sum_h E[s2][h] Y[sP][h], shared by the column pairs
             for (int sP = 0; sP < nsP; ++sP) {
                 const double* y = &w.Y[(size_t)sP * nhab];
-                double* a = &w.acc[(size_t)sP * nab];
+                double* v = &w.V[(size_t)sP * nab];
                 for (int s2 = 0; s2 < nab; ++s2) {
                     const double* e = &pp.E[(size_t)s2 * nhab];
-                    double v = 0.0;
-                    for (int h = 0; h < nhab; ++h) v += e[h] * y[h];
-                    a[s2] += pref * v;
+                    double t = 0.0;
+                    for (int h = 0; h < nhab; ++h) t += e[h] * y[h];
+                    v[s2] = t;
                 }
+            }
+            for (int c = 0; c < ncp; ++c) {
+                if (pp.w[c] == 0.0) continue;
+                const double pref = base * pp.w[c];
+                double* a = &w.acc[(size_t)c * nsP * nab];
+                for (int i = 0; i < nsP * nab; ++i) a[i] += pref * w.V[i];
             }
         }
     }
 }
 
+int max_aux_l(const f2ints_basis* aux, int sh0, int sh1) {
+    int m = 0;
+    for (int s = sh0; s < sh1; ++s) m = std::max(m, aux->shells[s].l);
+    return m;
+}
+
 int coulomb_3c(const f2ints_basis* aux, const f2ints_basis* bs, int sh0, int sh1, double* out) {
     const int64_t nbf = bs->nbf;
     const int64_t first = aux->shells[sh0].off;
-    const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
+    ket_pairs(bs);
     const int64_t nP = (sh1 < (int)aux->shells.size() ? aux->shells[sh1].off : aux->nbf) - first;
     std::fill(out, out + (size_t)nP * nbf * nbf, 0.0);
-    int maxlaux = 0;
-    for (int s = sh0; s < sh1; ++s) maxlaux = std::max(maxlaux, aux->shells[s].l);
+    const int maxlaux = max_aux_l(aux, sh0, sh1);
 #pragma omp parallel num_threads(nthreads())
     {
         Scratch3c w(maxlaux, bs->maxl);
 #pragma omp for schedule(dynamic)
-        for (size_t q = 0; q < pairs.size(); ++q) {
-            const Shell& A = bs->shells[pairs[q].first];
-            const Shell& B = bs->shells[pairs[q].second];
-            const int nsa = nsph(A.l), nsbb = nsph(B.l), nab = nsa * nsbb;
+        for (size_t k = 0; k < bs->fam_pairs.size(); ++k) {
+            const FamPair& f = *static_cast<const FamPair*>(bs->fam_pairs[k]);
+            const int nsa = nsph(bs->shells[f.a0].l), nsbb = nsph(bs->shells[f.b0].l), nab = nsa * nsbb;
             for (int s = sh0; s < sh1; ++s) {
                 const Shell& P = aux->shells[s];
-                eval_3c(bs, q, P, w);
-                for (int sP = 0; sP < nsph(P.l); ++sP) {
-                    double* o = out + (size_t)(P.off + sP - first) * nbf * nbf;
-                    for (int sa = 0; sa < nsa; ++sa)
-                        for (int sb = 0; sb < nsbb; ++sb) {
-                            const double v = w.acc[(size_t)sP * nab + sa * nsbb + sb];
-                            const int64_t m = A.off + sa, n = B.off + sb;
-                            o[m * nbf + n] = v;
-                            o[n * nbf + m] = v;
-                        }
+                const int nsP = nsph(P.l);
+                eval_fam(bs, f, P, w);
+                for (size_t c = 0; c < f.cols.size(); ++c) {
+                    if (f.q[c] < 0) continue;
+                    const Shell& A = bs->shells[f.a0 + f.cols[c].first];
+                    const Shell& B = bs->shells[f.b0 + f.cols[c].second];
+                    const double* acc = &w.acc[c * nsP * nab];
+                    for (int sP = 0; sP < nsP; ++sP) {
+                        double* o = out + (size_t)(P.off + sP - first) * nbf * nbf;
+                        for (int sa = 0; sa < nsa; ++sa)
+                            for (int sb = 0; sb < nsbb; ++sb) {
+                                const double v = acc[(size_t)sP * nab + sa * nsbb + sb];
+                                const int64_t m = A.off + sa, n = B.off + sb;
+                                o[m * nbf + n] = v;
+                                o[n * nbf + m] = v;
+                            }
+                    }
                 }
+            }
+        }
+    }
+    return 0;
+}
+
+// V[m][n] = sum over every aux function P of (P|mn), both triangles
+int coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out) {
+    const int64_t nbf = bs->nbf;
+    ket_pairs(bs);
+    std::fill(out, out + (size_t)nbf * nbf, 0.0);
+    const int maxlaux = max_aux_l(aux, 0, (int)aux->shells.size());
+#pragma omp parallel num_threads(nthreads())
+    {
+        Scratch3c w(maxlaux, bs->maxl);
+        std::vector<double> sum;
+#pragma omp for schedule(dynamic)
+        for (size_t k = 0; k < bs->fam_pairs.size(); ++k) {
+            const FamPair& f = *static_cast<const FamPair*>(bs->fam_pairs[k]);
+            const int nsa = nsph(bs->shells[f.a0].l), nsbb = nsph(bs->shells[f.b0].l), nab = nsa * nsbb;
+            const size_t ncp = f.cols.size();
+            sum.assign(ncp * nab, 0.0);
+            for (const Shell& P : aux->shells) {
+                const int nsP = nsph(P.l);
+                eval_fam(bs, f, P, w);
+                for (size_t c = 0; c < ncp; ++c)
+                    for (int sP = 0; sP < nsP; ++sP)
+                        for (int i = 0; i < nab; ++i) sum[c * nab + i] += w.acc[(c * nsP + sP) * nab + i];
+            }
+            // this family pair owns these entries: no other thread writes them
+            for (size_t c = 0; c < ncp; ++c) {
+                if (f.q[c] < 0) continue;
+                const Shell& A = bs->shells[f.a0 + f.cols[c].first];
+                const Shell& B = bs->shells[f.b0 + f.cols[c].second];
+                for (int sa = 0; sa < nsa; ++sa)
+                    for (int sb = 0; sb < nsbb; ++sb) {
+                        const int64_t m = A.off + sa, n = B.off + sb;
+                        out[m * nbf + n] = sum[c * nab + sa * nsbb + sb];
+                        out[n * nbf + m] = sum[c * nab + sa * nsbb + sb];
+                    }
             }
         }
     }
@@ -646,45 +809,12 @@ int64_t pair_ncols(const f2ints_basis* bs, const std::pair<int, int>& ab) {
     return ab.first == ab.second ? (int64_t)na * (na + 1) / 2 : (int64_t)na * nb;
 }
 
-// V[m][n] = sum over every aux function P of (P|mn), both triangles
-int coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out) {
-    const int64_t nbf = bs->nbf;
-    const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
-    std::fill(out, out + (size_t)nbf * nbf, 0.0);
-    int maxlaux = 0;
-    for (const Shell& P : aux->shells) maxlaux = std::max(maxlaux, P.l);
-#pragma omp parallel num_threads(nthreads())
-    {
-        Scratch3c w(maxlaux, bs->maxl);
-        std::vector<double> sum;
-#pragma omp for schedule(dynamic)
-        for (size_t q = 0; q < pairs.size(); ++q) {
-            const Shell& A = bs->shells[pairs[q].first];
-            const Shell& B = bs->shells[pairs[q].second];
-            const int nsa = nsph(A.l), nsbb = nsph(B.l), nab = nsa * nsbb;
-            sum.assign(nab, 0.0);
-            for (const Shell& P : aux->shells) {
-                eval_3c(bs, q, P, w);
-                for (int sP = 0; sP < nsph(P.l); ++sP)
-                    for (int k = 0; k < nab; ++k) sum[k] += w.acc[(size_t)sP * nab + k];
-            }
-            // pair q owns these entries: no other thread writes them
-            for (int sa = 0; sa < nsa; ++sa)
-                for (int sb = 0; sb < nsbb; ++sb) {
-                    const int64_t m = A.off + sa, n = B.off + sb;
-                    out[m * nbf + n] = sum[sa * nsbb + sb];
-                    out[n * nbf + m] = sum[sa * nsbb + sb];
-                }
-        }
-    }
-    return 0;
-}
-
 // (P|ab) for every aux function and the significant pairs [q0, q1), packed:
 // out[P * ldo + c], column c running over the pairs in order; a pair a > b
 // has nsa * nsb columns (row-major in (sa, sb)), a pair a == b the
 // nsa (nsa + 1) / 2 components sa >= sb (row-major), so every (m, n) with
-// m >= n appears once.
+// m >= n appears once.  The family pairs that hold these pairs are evaluated
+// whole; their other column pairs are not written.
 int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1, double* out,
                      int64_t ldo) {
     const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
@@ -692,28 +822,36 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0
     std::vector<int64_t> col(q1 - q0 + 1, 0);
     for (int64_t q = q0; q < q1; ++q) col[q - q0 + 1] = col[q - q0] + pair_ncols(bs, pairs[q]);
     if (ldo < col.back()) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the block's columns");
-    int maxlaux = 0;
-    for (const Shell& P : aux->shells) maxlaux = std::max(maxlaux, P.l);
+    std::vector<int> fams;
+    for (int64_t q = q0; q < q1; ++q) fams.push_back(bs->fam_of_pair[q]);
+    std::sort(fams.begin(), fams.end());
+    fams.erase(std::unique(fams.begin(), fams.end()), fams.end());
+    const int maxlaux = max_aux_l(aux, 0, (int)aux->shells.size());
 #pragma omp parallel num_threads(nthreads())
     {
         Scratch3c w(maxlaux, bs->maxl);
 #pragma omp for schedule(dynamic)
-        for (int64_t q = q0; q < q1; ++q) {
-            const int na = nsph(bs->shells[pairs[q].first].l), nb = nsph(bs->shells[pairs[q].second].l);
-            const int nab = na * nb;
-            const bool diag = pairs[q].first == pairs[q].second;
-            double* o = out + col[q - q0];
+        for (size_t k = 0; k < fams.size(); ++k) {
+            const FamPair& f = *static_cast<const FamPair*>(bs->fam_pairs[fams[k]]);
+            const int na = nsph(bs->shells[f.a0].l), nb = nsph(bs->shells[f.b0].l), nab = na * nb;
             for (const Shell& P : aux->shells) {
-                eval_3c(bs, (size_t)q, P, w);
-                for (int sP = 0; sP < nsph(P.l); ++sP) {
-                    double* row = o + (size_t)(P.off + sP) * ldo;
-                    const double* a = &w.acc[(size_t)sP * nab];
-                    if (!diag) {
-                        std::copy(a, a + nab, row);
-                    } else {
-                        int c = 0;
-                        for (int sa = 0; sa < na; ++sa)
-                            for (int sb = 0; sb <= sa; ++sb) row[c++] = a[sa * nb + sb];
+                const int nsP = nsph(P.l);
+                eval_fam(bs, f, P, w);
+                for (size_t c = 0; c < f.cols.size(); ++c) {
+                    const int64_t q = f.q[c];
+                    if (q < q0 || q >= q1) continue;
+                    const bool diag = pairs[q].first == pairs[q].second;
+                    double* o = out + col[q - q0];
+                    for (int sP = 0; sP < nsP; ++sP) {
+                        double* row = o + (size_t)(P.off + sP) * ldo;
+                        const double* a = &w.acc[(c * nsP + sP) * nab];
+                        if (!diag) {
+                            std::copy(a, a + nab, row);
+                        } else {
+                            int t = 0;
+                            for (int sa = 0; sa < na; ++sa)
+                                for (int sb = 0; sb <= sa; ++sb) row[t++] = a[sa * nb + sb];
+                        }
                     }
                 }
             }
@@ -784,7 +922,7 @@ int coulomb_4c(const f2ints_basis* bs, double* out) {
 }  // namespace
 
 f2ints_basis::~f2ints_basis() {
-    for (void* p : pair_data) delete static_cast<ShellPair*>(p);
+    for (void* p : fam_pairs) delete static_cast<FamPair*>(p);
 }
 
 // =========================================================================== C ABI
