@@ -677,13 +677,14 @@ void eval_fam(const f2ints_basis* bs, const FamPair& f, const Shell& P, Scratch3
     const int lP = P.l, nsP = nsph(lP), L = lP + lab;
     const int ncp = (int)f.cols.size();
     w.acc.assign((size_t)ncp * nsP * nab, 0.0);
+    static const double two_pi52 = 2.0 * std::pow(PI, 2.5);
     for (size_t k = 0; k < P.alpha.size(); ++k) {
         const double g = P.alpha[k];
+        const double gl = std::pow(2.0 * g, -lP);  // once per aux primitive
         for (const FamPrim& pp : f.prims) {
             const double p = pp.p, al = g * p / (g + p);
             compute_R(L, al, P.A[0] - pp.P[0], P.A[1] - pp.P[1], P.A[2] - pp.P[2], w.R.data(), w.work.data());
-            const double base = 2.0 * std::pow(PI, 2.5) / (g * p * std::sqrt(g + p)) * P.coef[k] *
-                                std::pow(2.0 * g, -lP);
+            const double base = two_pi52 / (g * p * std::sqrt(g + p)) * P.coef[k] * gl;
             // Y[sP][h] = sum_c T_{lP}(sP, c) R[c + h]
             for (int sP = 0; sP < nsP; ++sP) {
                 double* y = &w.Y[(size_t)sP * nhab];
