@@ -30,7 +30,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 # with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size:
 # x2 and x1) on these kernels and launch shapes (profiles/pmc_traffic_h2o64_bf16x3.md);
 # other workloads report null.
-MEASURED_KX_TRAFFIC = {(64, "bf16x3"): 18.42e9 + 3.64e9}
+MEASURED_KX_TRAFFIC = {(64, "bf16x3"): 18.72e9 + 3.62e9}
 HBM_PEAK_GBS = 8000.0
 
 
