@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--serve-nmax", type=int, default=None, help="largest cluster/chain (24 water, 20 alkane)")
     ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--serve-slots", type=int, default=12, help="jobs in flight")
+    ap.add_argument("--serve-integral-threads", type=int, default=0,
+                    help="threads of each host integral call (0: all host cores; concurrent "
+                         "preparations share them)")
     ap.add_argument("--serve-prep-threads", type=int, default=2,
                     help="host threads preparing admitted jobs (integral threads = cores / this)")
     return ap.parse_args()
@@ -161,7 +164,7 @@ def serve_mode(args):
         workload = (f"{len(reqs)} UHF requests, C_nH_(2n+2) n~U{{2..{nmax}}} ({ncat} cations, doublet; "
                     f"neutrals ms=0), cc-pVTZ/cc-pVTZ-JKFIT, Poisson arrivals at {args.serve_rate}/s (seed 11)")
     cores = host_cores()
-    ithreads = max(1, cores // args.serve_prep_threads)
+    ithreads = args.serve_integral_threads or cores
     integrals.set_threads(ithreads)
     last = [0.0]
 
