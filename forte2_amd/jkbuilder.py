@@ -312,21 +312,34 @@ class FockBuilder:
             return J[0].cpu().numpy(), K[0].cpu().numpy()
         return J[0], K[0]
 
+    # The assembly methods below call build_JK / build_JK_generalized with the
+    # reference's own arguments when no precision is given, so a builder whose
+    # build methods are replaced (as forte2's tests do) assembles the same way.
     def build_core_fock(self, C_core, hcore=None, precision=None):
-        """F_core = h + 2 J[C_core] - K[C_core] (jkbuilder.py:292-321)."""
+        """F_core = h + 2 J[C_core] - K[C_core], h + J - K for two-component
+        systems (jkbuilder.py:292-321)."""
         if hcore is None:
             hcore = self.system.ints_hcore()
-        J, K = self.build_JK([C_core], precision)
+        J, K = self.build_JK([C_core]) if precision is None else self.build_JK([C_core], precision=precision)
+        if getattr(self.system, "two_component", False):
+            return hcore + J[0] - K[0]
         return hcore + 2.0 * J[0] - K[0]
 
     def build_active_fock(self, C_act, g1, precision=None):
-        """F_act = J[g1] - K[g1] / 2 (jkbuilder.py:323-349)."""
-        J, K = self.build_JK_generalized(C_act, g1, precision)
+        """F_act = J[g1] - K[g1] / 2, J - K for two-component systems
+        (jkbuilder.py:323-349)."""
+        if precision is None:
+            J, K = self.build_JK_generalized(C_act, g1)
+        else:
+            J, K = self.build_JK_generalized(C_act, g1, precision=precision)
+        if getattr(self.system, "two_component", False):
+            return J - K
         return J - 0.5 * K
 
     def build_generalized_fock(self, C_core, C_act, g1, hcore=None, precision=None):
         """F = F_core + F_act (jkbuilder.py:351-380)."""
-        return self.build_core_fock(C_core, hcore, precision) + self.build_active_fock(C_act, g1, precision)
+        return (self.build_core_fock(C_core, hcore=hcore, precision=precision)
+                + self.build_active_fock(C_act, g1, precision=precision))
 
     # ------------------------------------------------------------------ MO-basis blocks
     def half_transform(self, C, p0=0, p1=None, out=None):

@@ -309,7 +309,13 @@ class SCFServer:
                 # admission in arrival order: slot and HBM budget
                 while queue and len(active) + len(preparing) < self.max_active:
                     req = queue[0]
-                    fp = self._estimate(req)
+                    try:
+                        fp = self._estimate(req)
+                    except Exception as e:  # unreadable request (geometry, basis name): this job only
+                        queue.pop(0)
+                        self.results[req.id] = SCFResult(req.id, arrival=req.arrival, admitted=now(),
+                                                         finished=now(), error=repr(e))
+                        continue
                     if fp > self.hbm_budget:  # can never fit: rejected, the stream goes on
                         queue.pop(0)
                         self.results[req.id] = SCFResult(req.id, arrival=req.arrival, admitted=now(),

@@ -154,3 +154,25 @@ def test_precision_form_in_the_mix(dev):
     for r in reqs:
         assert mix[r.id].converged
         assert abs(mix[r.id].E - ref[r.id].E) <= 1e-8
+
+
+def test_failures_are_isolated(dev):
+    # a request with an unknown basis, one with a negative electron count and
+    # one too large for the HBM budget fail on their own; the others finish
+    from forte2_amd.serving import job_footprint
+
+    w1, w2, w8 = (geo.water_cluster(n, seed=200 + n) for n in (1, 2, 8))
+    reqs = [SCFRequest(0, w1, arrival=0.0),
+            SCFRequest(1, w1, basis="no-such-basis", arrival=0.0),
+            SCFRequest(2, w1, charge=20, arrival=0.0),
+            SCFRequest(3, w2, arrival=0.0),
+            SCFRequest(4, w8, arrival=0.0)]
+    srv = SCFServer(device=dev, hbm_budget=int(1.5 * job_footprint(48, 278, 20)), e_tol=1e-8)
+    res = srv.run(reqs)
+    assert res[0].converged and res[3].converged
+    for i in (1, 2, 4):
+        assert res[i].error, i
+        print(f"   request {i}: {res[i].error[:80]}", flush=True)
+    assert "HBM budget" in res[4].error
+    alone = SCFServer(device=dev, e_tol=1e-8).run([SCFRequest(3, w2, arrival=0.0)])
+    assert res[3].E == alone[3].E
