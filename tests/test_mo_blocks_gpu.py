@@ -113,3 +113,23 @@ def test_mp2_golden(dev):
     emp2 = scf.E + np.sum(V * (2.0 * V - V.transpose(0, 1, 3, 2)) / den)
     print(f"   E(MP2) = {emp2:.12f}", flush=True)
     assert emp2 == pytest.approx(-76.3710978841482984, abs=5e-8)
+
+
+def test_ao_jk_equal_mo_reconstruction(dev):
+    # tests/jkbuilder/test_jkbuilder.py:56-85: J and K of the occupied density
+    # rebuilt from the MO-basis <pq|rs> of the full orbital set agree with the
+    # AO J/K builds (two independent device paths: MFMA half-transform and
+    # contraction vs MO blocks by GEMMs)
+    Q, N, no = 300, 40, 7
+    B = synthetic_B(Q, N, seed=21, device=dev)
+    fb = FockBuilder(B=B, device=dev)
+    C = orthonormal_C(N, N, seed=22, device=dev)
+    J, K = fb.build_JK([C[:, :no]])
+    V = fb.two_electron_integrals_block(C)          # <pq|rs> = (pr|qs)
+    occ = slice(0, no)
+    J_mo = torch.einsum("piqi->pq", V[:, occ, :, occ].contiguous())
+    K_mo = torch.einsum("piiq->pq", V[:, occ, occ, :].contiguous())
+    ej = float((C.T @ J[0] @ C - J_mo).abs().max())
+    ek = float((C.T @ K[0] @ C - K_mo).abs().max())
+    print(f"   |dJ_mo|={ej:.2e} |dK_mo|={ek:.2e}", flush=True)
+    assert ej <= TOL and ek <= TOL
