@@ -12,7 +12,9 @@ the launch shape:
   * ``bf16_nbw_max`` (BF16x3): the widest half-transform tile, 32*NBW
     columns.  Narrower tiles use fewer registers and LDS but re-read the B
     rows once per column tile;
-  * ``kx_waves`` (BF16x3): 4 or 8 waves per half-transform workgroup.
+  * ``kx_waves`` (BF16x3): 4 or 8 waves per half-transform workgroup;
+  * ``ks_tile`` (BF16x3, single-job): 128 or 256, the lower-tile edge of the
+    contraction.
 
 ``tune(shapes)`` times every candidate on synthetic B of each shape with HIP
 events and keeps the fastest; ``tune_grouped(shapes)`` does the same for a
@@ -41,10 +43,11 @@ CANDIDATES = {
     "xchunk_bytes": [1 << 29, 1 << 30, 1 << 31, 1 << 32],
     "bf16_nbw_max": [4, 5, 6, 8, 10],
     "kx_waves": [4, 8],
+    "ks_tile": [128, 256],
 }
 # the BF16x3 grid is larger (tile cap, waves): fewer chunk sizes there
 XCHUNK_BF16 = [1 << 31, 1 << 32]
-KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max", "kx_waves")
+KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max", "kx_waves", "ks_tile")
 
 
 def _key(nbf, naux, precision, grouped=False):
@@ -59,8 +62,9 @@ def _candidates(precision, grouped):
     names = ["target_blocks"] + ([] if grouped else ["xchunk_bytes"])
     if precision == "bf16x3":
         names.append("bf16_nbw_max")
-        if not grouped:  # grouped half-transforms use 4-wave workgroups
+        if not grouped:  # grouped builds use 4-wave half-transform and 128-wide contraction tiles
             names.append("kx_waves")
+            names.append("ks_tile")
     pools = {**CANDIDATES, "xchunk_bytes": XCHUNK_BF16 if precision == "bf16x3" else CANDIDATES["xchunk_bytes"]}
     for vals in itertools.product(*[pools[n] for n in names]):
         yield dict(zip(names, vals))
@@ -226,12 +230,14 @@ if __name__ == "__main__":
     ap.add_argument("--merge", action="store_true",
                     help="update the entries of the existing table at --out instead of replacing it")
     ap.add_argument("--skip-grouped", action="store_true", help="single-job entries only")
+    ap.add_argument("--skip-single", action="store_true", help="grouped entries only")
     a = ap.parse_args()
     log = lambda m: print(m, flush=True)
     precs = tuple(a.precisions.split(","))
     ints = lambda s: [int(x) for x in s.split(",") if x]
     table = load(a.out) if a.merge else {}
-    table.update(tune(water_shapes(ints(a.waters)) + alkane_shapes(ints(a.alkanes)), precs, log=log))
+    if not a.skip_single:
+        table.update(tune(water_shapes(ints(a.waters)) + alkane_shapes(ints(a.alkanes)), precs, log=log))
     if not a.skip_grouped:
         table.update(tune_grouped(water_shapes(ints(a.grouped_waters)) + alkane_shapes(ints(a.alkanes)), precs,
                                   group=a.group, log=log))

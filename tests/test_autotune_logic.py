@@ -54,6 +54,7 @@ def test_candidate_sets():
     grouped_bf = list(autotune._candidates("bf16x3", grouped=True))
     assert len(grouped_bf) == 35 and all("xchunk_bytes" not in c and "kx_waves" not in c for c in grouped_bf)
     single_bf = list(autotune._candidates("bf16x3", grouped=False))
-    assert len(single_bf) == 140 and {c["kx_waves"] for c in single_bf} == {4, 8}
+    assert len(single_bf) == 280 and {c["kx_waves"] for c in single_bf} == {4, 8}
+    assert {c["ks_tile"] for c in single_bf} == {128, 256}
     N, Q, o = autotune.alkane_shapes([20])[0]
     assert (N, Q, o) == (1188, 2840, [81, 80])
