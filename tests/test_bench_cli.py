@@ -1,0 +1,25 @@
+"""bench.py command-line contract on the CPU: one GPU only, a JSON line and
+exit status 2 for any other GPU count."""
+
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    line = json.loads(p.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_help_lists_the_modes():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 0
+    for flag in ("--steps", "--warmup", "--gpus", "--serve", "--otf", "--scf"):
+        assert flag in p.stdout
