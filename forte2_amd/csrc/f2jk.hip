@@ -989,20 +989,27 @@ int f2jk_half_transform(f2jk_ctx* ctx, const double* C, int64_t ncol, int64_t ld
     const double* cp = (const double*)(w + p.off_cp);
     double* Xc = (double*)(w + p.off_x);
     const int64_t NN = (int64_t)p.N * p.N;
+    // the kernel writes rows of OPAD columns: straight into the caller's X
+    // when its row stride is exactly that, through the workspace otherwise
+    const bool direct = (ldx == p.OPAD);
     for (int64_t P0 = p0; P0 < p1; P0 += p.Pc) {
         const int pc = (int)std::min<int64_t>(p.Pc, p1 - P0);
         const int64_t R = (int64_t)pc * p.N;
         const unsigned grid = (unsigned)(cdiv(R, kTile) * p.ncolt);
+        double* dst = X + (P0 - p0) * p.N * ldx;
+        double* Xo = direct ? dst : Xc;
         if (p.kva == 2)
             hipLaunchKernelGGL((kx_kernel<double, 2>), dim3(grid), dim3(kGThreads), 0, st, B + (int64_t)P0 * NN, R,
-                               p.N, cp, p.OPAD, Xc, p.ncolt);
+                               p.N, cp, p.OPAD, Xo, p.ncolt);
         else
             hipLaunchKernelGGL((kx_kernel<double, 1>), dim3(grid), dim3(kGThreads), 0, st, B + (int64_t)P0 * NN, R,
-                               p.N, cp, p.OPAD, Xc, p.ncolt);
+                               p.N, cp, p.OPAD, Xo, p.ncolt);
         F2JK_LAUNCHED("kx");
-        hipLaunchKernelGGL(x_copy_kernel, dim3((unsigned)cdiv(R * ncol, 256)), dim3(256), 0, st, Xc, R, p.OPAD,
-                           (int)ncol, X + (P0 - p0) * p.N * ldx, ldx);
-        F2JK_LAUNCHED("x_copy");
+        if (!direct) {
+            hipLaunchKernelGGL(x_copy_kernel, dim3((unsigned)cdiv(R * ncol, 256)), dim3(256), 0, st, Xc, R,
+                               p.OPAD, (int)ncol, dst, ldx);
+            F2JK_LAUNCHED("x_copy");
+        }
     }
     g_last_error.clear();
     return F2JK_OK;

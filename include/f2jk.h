@@ -169,7 +169,8 @@ int f2jk_group_build_jk(f2jk_ctx* const* ctxs, int njobs, const int* ndens,
  * FockBuilder.B_tensor_gen_block (forte2/jkbuilder/jkbuilder.py:521-552),
  * whose einsum "Pmn,mi,nj->Pij" is finished by a GEMM with C1 on the caller's
  * side; two_electron_integrals_gen_block (:382-420) contracts two such blocks
- * over P.  Workspace: f2jk_workspace_size(ctx, 1, &ncol, FP64, F2JK_WANT_K). */
+ * over P.  Workspace: f2jk_workspace_size(ctx, 1, &ncol, FP64, F2JK_WANT_K).  With ldx equal
+ * to ncol rounded up to 16 the kernel writes X in place (no copy). */
 int f2jk_half_transform(f2jk_ctx* ctx, const double* C, int64_t ncol, int64_t ldc, int64_t p0,
                         int64_t p1, double* X, int64_t ldx, int precision, void* ws, size_t ws_bytes,
                         void* stream);

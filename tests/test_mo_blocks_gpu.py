@@ -24,18 +24,20 @@ def _err(a, b):
     return float(np.abs(a - b).max())
 
 
-@pytest.mark.parametrize("Q,N", [(300, 131), (139, 24)])
-def test_half_transform_blocks(dev, Q, N):
-    print(f"[mo-gpu] half-transform Q={Q} N={N}", flush=True)
+@pytest.mark.parametrize("Q,N,ncol", [(300, 131, 21), (139, 24, 21), (300, 131, 32)])
+def test_half_transform_blocks(dev, Q, N, ncol):
+    # 21 columns: a column slice (row stride 30), written through the
+    # workspace; 32 columns: X rows as wide as the kernel's, written in place
+    print(f"[mo-gpu] half-transform Q={Q} N={N} ncol={ncol}", flush=True)
     B = synthetic_B(Q, N, seed=Q, device=dev)
     Bh = B.cpu().numpy()
-    wide = orthonormal_C(N, 30, seed=3, device=dev)
-    C = wide[:, 4:25]                      # column slice: row stride 30
+    wide = orthonormal_C(N, 32, seed=3, device=dev)
+    C = wide[:, 4:25] if ncol == 21 else wide
     fb = FockBuilder(B=B, device=dev)
     fb.set_tuning("xchunk_bytes", 1 << 20)  # several kernel chunks per block
     Xr = np.einsum("Pmn,ni->Pmi", Bh, C.cpu().numpy())
     seen = 0
-    for P0, P1, X in fb._x_blocks(C, block_bytes=8 * N * 21 * 37):  # 37 P per block
+    for P0, P1, X in fb._x_blocks(C, block_bytes=8 * N * ncol * 37):  # 37 P per block
         assert P0 == seen
         seen = P1
         assert _err(X, Xr[P0:P1]) <= TOL
