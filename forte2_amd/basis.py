@@ -14,6 +14,7 @@ import itertools
 import json
 import math
 import os
+import threading
 
 import numpy as np
 
@@ -88,22 +89,31 @@ class BasisSet:
 
     @property
     def handle(self):
-        """f2ints_basis* of the host integral engine (created on first use)."""
+        """f2ints_basis* of the host integral engine (created on first use;
+        safe when threads ask for it at once: one handle per basis)."""
         if self._handle is None:
-            lib = _native.ints_lib()
-            ls = (ctypes.c_int * self.nshells)(*[s[0] for s in self.shells])
-            nps = (ctypes.c_int * self.nshells)(*[len(s[1]) for s in self.shells])
-            xyz = np.ascontiguousarray(np.concatenate([s[3] for s in self.shells]) if self.shells else np.zeros(0))
-            ex = np.ascontiguousarray(np.concatenate([s[1] for s in self.shells]) if self.shells else np.zeros(0))
-            co = np.ascontiguousarray(np.concatenate([s[2] for s in self.shells]) if self.shells else np.zeros(0))
-            h = ctypes.c_void_p()
-            dp = ctypes.POINTER(ctypes.c_double)
-            rc = lib.f2ints_basis_create(self.nshells, ls, nps, xyz.ctypes.data_as(dp), ex.ctypes.data_as(dp),
-                                         co.ctypes.data_as(dp), ctypes.byref(h))
-            if rc != 0:
-                raise ValueError(lib.f2ints_last_error().decode())
-            self._handle = _Handle(h)
+            with _HANDLE_LOCK:
+                if self._handle is None:
+                    self._make_handle()
         return self._handle.ptr
+
+    def _make_handle(self):
+        lib = _native.ints_lib()
+        ls = (ctypes.c_int * self.nshells)(*[s[0] for s in self.shells])
+        nps = (ctypes.c_int * self.nshells)(*[len(s[1]) for s in self.shells])
+        xyz = np.ascontiguousarray(np.concatenate([s[3] for s in self.shells]) if self.shells else np.zeros(0))
+        ex = np.ascontiguousarray(np.concatenate([s[1] for s in self.shells]) if self.shells else np.zeros(0))
+        co = np.ascontiguousarray(np.concatenate([s[2] for s in self.shells]) if self.shells else np.zeros(0))
+        h = ctypes.c_void_p()
+        dp = ctypes.POINTER(ctypes.c_double)
+        rc = lib.f2ints_basis_create(self.nshells, ls, nps, xyz.ctypes.data_as(dp), ex.ctypes.data_as(dp),
+                                     co.ctypes.data_as(dp), ctypes.byref(h))
+        if rc != 0:
+            raise ValueError(lib.f2ints_last_error().decode())
+        self._handle = _Handle(h)
+
+
+_HANDLE_LOCK = threading.Lock()
 
 
 class _Handle:

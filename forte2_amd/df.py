@@ -90,9 +90,11 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
     one of two pinned staging buffers.  Meanwhile the device copies the
     previous block in, multiplies it by X (a (Q, Q) x (Q, ncols) GEMM, half
     the flops of the full square) and scatters the columns to (m, n) and
-    (n, m) of B.  `stats`, if a dict, receives the total seconds, the
-    seconds the host engine spent (overlapped with the device work), the
-    number of blocks and of packed columns."""
+    (n, m) of B.  The first block is evaluated while the metric is
+    factored.  `stats`, if a dict, receives the total seconds, the seconds
+    the host engine spent on its worker thread (overlapped with the rest),
+    the number of blocks and of packed columns, and the times to the metric,
+    the fitting matrix and the end of the pipeline."""
     import concurrent.futures
     import time
 
@@ -103,10 +105,6 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
         raise ValueError("Auxiliary basis is not defined. Define auxiliary_basis_set.")
     Q, N = aux.size, bas.size
     t_start = time.perf_counter()
-    M = torch.from_numpy(_ints.coulomb_2c(aux)).to(device)
-    kind, fac = metric_factor(M, getattr(system, "df_ortho_rtol", None))
-    X = _fitting_matrix(kind, fac)
-    del M, fac
     col0, mcol, ncol_ = pair_columns(bas)
     npairs, ncols = len(col0) - 1, int(col0[-1])
     # pair blocks of at most block_bytes of packed columns
@@ -118,16 +116,9 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
             q1 += 1
         blocks.append((q, q1, int(col0[q]), int(col0[q1])))
         q = q1
-    B = torch.zeros((Q, N, N), dtype=torch.float64, device=device)
-    if not blocks:
-        return B
-    wmax = max(c1 - c0 for _, _, c0, c1 in blocks)
-    idx_mn = torch.from_numpy(mcol * N + ncol_).to(device)
-    idx_nm = torch.from_numpy(ncol_ * N + mcol).to(device)
-    Bv = B.view(Q, N * N)
     pin = torch.device(device).type == "cuda"
+    wmax = max((c1 - c0 for _, _, c0, c1 in blocks), default=1)
     host = [torch.empty(Q * wmax, dtype=torch.float64, pin_memory=pin) for _ in range(2)]
-    dev_in = torch.empty(Q * wmax, dtype=torch.float64, device=device)
     host_s = [0.0]
 
     def packed(flat, w):
@@ -139,10 +130,23 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
         _ints.coulomb_3c_pairs(aux, bas, q0, q1, packed(host[slot], c1 - c0).numpy())
         host_s[0] += time.perf_counter() - t0
 
-    stream = torch.cuda.current_stream(device) if pin else None
-    copied = [None, None]
+    aux.handle, bas.handle  # engine handles made here, before two threads use them
     with concurrent.futures.ThreadPoolExecutor(max_workers=1) as pool:
-        fut = pool.submit(evaluate, 0, 0)
+        # the first pair block is evaluated while the metric is factored
+        fut = pool.submit(evaluate, 0, 0) if blocks else None
+        M = torch.from_numpy(_ints.coulomb_2c(aux)).to(device)
+        t_m2c = time.perf_counter()
+        kind, fac = metric_factor(M, getattr(system, "df_ortho_rtol", None))
+        X = _fitting_matrix(kind, fac)
+        del M, fac
+        t_fit = time.perf_counter()
+        B = torch.zeros((Q, N, N), dtype=torch.float64, device=device)
+        Bv = B.view(Q, N * N)
+        idx_mn = torch.from_numpy(mcol * N + ncol_).to(device)
+        idx_nm = torch.from_numpy(ncol_ * N + mcol).to(device)
+        dev_in = torch.empty(Q * wmax, dtype=torch.float64, device=device)
+        stream = torch.cuda.current_stream(device) if pin else None
+        copied = [None, None]
         for k, (q0, q1, c0, c1) in enumerate(blocks):
             slot = k % 2
             fut.result()
@@ -167,7 +171,8 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
         torch.cuda.synchronize(device)
     if stats is not None:
         stats.update(total_s=time.perf_counter() - t_start, host_s=host_s[0], blocks=len(blocks),
-                     packed_columns=ncols)
+                     packed_columns=ncols, metric_2c_s=t_m2c - t_start, fitting_matrix_s=t_fit - t_m2c,
+                     pipeline_s=time.perf_counter() - t_fit)
     return B
 
 

@@ -181,3 +181,25 @@ def test_three_center_sum_matches_blocks():
     V = ints.coulomb_3c_sum(dens, b)
     assert np.abs(V - ref).max() < 1e-12 * np.abs(ref).max()
     assert np.array_equal(V, V.T)
+
+
+def test_engine_handle_is_made_once_across_threads():
+    # threads asking a fresh basis for its engine handle at once get the same
+    # one (a second handle would free the first while it is in use)
+    import threading
+
+    atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5")
+    for _ in range(5):
+        b = bs.build_basis("cc-pvtz-jkfit", atoms)
+        got, start = [], threading.Barrier(8)
+
+        def ask():
+            start.wait()
+            got.append(b.handle.value)
+
+        ts = [threading.Thread(target=ask) for _ in range(8)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert len(set(got)) == 1
