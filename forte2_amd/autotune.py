@@ -58,13 +58,17 @@ def _noccs(o):
     return list(o) if isinstance(o, (list, tuple)) else [int(o)]
 
 
-def _candidates(precision, grouped):
+KS_TILE_MIN_NBF = 512  # below this a 256-wide contraction tile is mostly empty: not offered
+
+
+def _candidates(precision, grouped, nbf=None):
     names = ["target_blocks"] + ([] if grouped else ["xchunk_bytes"])
     if precision == "bf16x3":
         names.append("bf16_nbw_max")
         if not grouped:  # grouped builds use 4-wave half-transform and 128-wide contraction tiles
             names.append("kx_waves")
-            names.append("ks_tile")
+            if nbf is None or nbf >= KS_TILE_MIN_NBF:
+                names.append("ks_tile")
     pools = {**CANDIDATES, "xchunk_bytes": XCHUNK_BF16 if precision == "bf16x3" else CANDIDATES["xchunk_bytes"]}
     for vals in itertools.product(*[pools[n] for n in names]):
         yield dict(zip(names, vals))
@@ -90,7 +94,7 @@ def tune(shapes, precisions=("fp64",), reps=3, device=None, log=print):
         fb = FockBuilder(B=B, device=dev)
         for prec in precisions:
             best = None
-            for cand in _candidates(prec, grouped=False):
+            for cand in _candidates(prec, grouped=False, nbf=N):
                 _set(fb, cand)
                 fb.timing = False
                 fb.build_JK(Cs, precision=prec)  # warm-up (workspace of this plan)

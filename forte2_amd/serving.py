@@ -162,6 +162,9 @@ class SCFServer:
         autotune.apply(fb, self.precision, grouped=self.grouped)
         if self.grouped:
             fb.set_tuning("xchunk_bytes", 1 << 32)
+            # its builds that run alone contract in the grouped tile (128), so
+            # a job's K does not depend on which path a step took
+            fb.set_tuning("ks_tile", 128)
         system.fock_builder = fb
         kw = dict(e_tol=self.e_tol, d_tol=self.d_tol, maxiter=self.maxiter, precision=self.precision)
         scf = RHF(req.charge, **kw) if req.ms is None else UHF(req.charge, ms=req.ms, **kw)

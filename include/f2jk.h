@@ -153,8 +153,9 @@ int f2jk_work(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
  * group); flags: WANT_J|WANT_K (J from the C of each density) or WANT_K alone.
  * A job's results do not depend on what else is in the group (bitwise).  In
  * FP64 they are bitwise those of f2jk_build_jk.  In the precision forms K
- * is bitwise that of f2jk_build_jk, and J differs from it in summation order
- * (the group reads B with a 2-wide vector).  nbf even; the job's X must fit
+ * is bitwise that of f2jk_build_jk with the same target_blocks and ks_tile
+ * 128 (grouped builds contract in 128-wide tiles), and J differs from it in
+ * summation order (the group reads B with a 2-wide vector).  nbf even; the job's X must fit
  * one chunk (EUNSUPPORTED otherwise). */
 int f2jk_group_workspace_size(f2jk_ctx* const* ctxs, int njobs, const int* ndens,
                               const int64_t* nocc, int precision, int flags,

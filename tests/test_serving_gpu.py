@@ -25,6 +25,7 @@ def test_grouped_build_is_bitwise_single_build(dev):
     for i, (Q, N, noccs) in enumerate(shapes):
         fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=i, device=dev), device=dev))
         fbs[-1].set_tuning("target_blocks", 2048)  # one plan for grouped and single builds
+        fbs[-1].set_tuning("ks_tile", 128)  # the grouped contraction tile
         Cs.append([orthonormal_C(N, o, seed=10 * i + k, device=dev) for k, o in enumerate(noccs)])
     singles = [fb.build_JK(C) for fb, C in zip(fbs, Cs)]
     srv = SCFServer(device=dev)
@@ -72,6 +73,7 @@ def test_grouped_precision_forms(dev, prec, kind):
     for i, (Q, N, noccs) in enumerate(shapes):
         fbs.append(FockBuilder(B=synthetic_B(Q, N, seed=i, device=dev), device=dev))
         fbs[-1].set_tuning("target_blocks", 2048)  # one plan for grouped and single builds
+        fbs[-1].set_tuning("ks_tile", 128)  # the grouped contraction tile
         Cs.append([orthonormal_C(N, o, seed=10 * i + k, device=dev) for k, o in enumerate(noccs)])
     srv = SCFServer(device=dev)
     for nd in (1, 2):
