@@ -206,10 +206,49 @@ const HermIndex& hidx() {
     return h;
 }
 
+// One step of the Hermite recursion R^n_{tuv} = P_d R^{n+1}_{t-1..} + (k-1) R^{n+1}_{t-2..}
+// along the first non-zero direction d of (t, u, v), with its source positions
+// in hidx().list (b < 0: no second term).  The order is that of list[LM], so
+// the steps of any L are a prefix.
+struct RStep {
+    int N, dir, a, b;
+    double k1;  // the (k - 1) factor of the second term
+};
+const std::vector<RStep>& r_steps() {
+    static const std::vector<RStep> steps = [] {
+        const HermIndex& H = hidx();
+        std::vector<RStep> out;
+        const auto& lst = H.list[H.LM];
+        for (size_t i = 0; i < lst.size(); ++i) {
+            const int t = lst[i][0], u = lst[i][1], v = lst[i][2];
+            RStep st{t + u + v, 0, -1, -1, 0.0};
+            if (i == 0) {
+                out.push_back(st);
+                continue;
+            }
+            if (t > 0) {
+                st.dir = 0;
+                st.a = H.at(t - 1, u, v);
+                if (t > 1) st.b = H.at(t - 2, u, v), st.k1 = t - 1;
+            } else if (u > 0) {
+                st.dir = 1;
+                st.a = H.at(t, u - 1, v);
+                if (u > 1) st.b = H.at(t, u - 2, v), st.k1 = u - 1;
+            } else {
+                st.dir = 2;
+                st.a = H.at(t, u, v - 1);
+                if (v > 1) st.b = H.at(t, u, v - 2), st.k1 = v - 1;
+            }
+            out.push_back(st);
+        }
+        return out;
+    }();
+    return steps;
+}
+
 // R^0_{tuv}(alpha, X, Y, Z) for t+u+v <= L, stored by position in hidx().list[L].
 // work must hold (L+1) * nherm(L) doubles.
 void compute_R(int L, double alpha, double X, double Y, double Z, double* R0, double* work) {
-    const HermIndex& H = hidx();
     const int nh = nherm(L);
     double F[NBOYS + 2];
     boys().eval(L, alpha * (X * X + Y * Y + Z * Z), F);
@@ -219,23 +258,15 @@ void compute_R(int L, double alpha, double X, double Y, double Z, double* R0, do
         work[(size_t)n * nh + 0] = m2a * F[n];
         m2a *= -2.0 * alpha;
     }
-    const auto& lst = H.list[L];
+    const RStep* steps = r_steps().data();
+    const double P[3] = {X, Y, Z};
     for (int i = 1; i < nh; ++i) {
-        const int t = lst[i][0], u = lst[i][1], v = lst[i][2];
-        const int N = t + u + v;
-        for (int n = 0; n <= L - N; ++n) {
+        const RStep& st = steps[i];
+        const double Pd = P[st.dir];
+        for (int n = 0; n <= L - st.N; ++n) {
             const double* Rn1 = work + (size_t)(n + 1) * nh;
-            double val;
-            if (t > 0) {
-                val = X * Rn1[H.at(t - 1, u, v)];
-                if (t > 1) val += (t - 1) * Rn1[H.at(t - 2, u, v)];
-            } else if (u > 0) {
-                val = Y * Rn1[H.at(t, u - 1, v)];
-                if (u > 1) val += (u - 1) * Rn1[H.at(t, u - 2, v)];
-            } else {
-                val = Z * Rn1[H.at(t, u, v - 1)];
-                if (v > 1) val += (v - 1) * Rn1[H.at(t, u, v - 2)];
-            }
+            double val = Pd * Rn1[st.a];
+            if (st.b >= 0) val += st.k1 * Rn1[st.b];
             work[(size_t)n * nh + i] = val;
         }
     }
@@ -549,7 +580,7 @@ int coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* out) {
 // them with its own coefficient products.
 struct FamPrim {
     double p, P[3];
-    std::vector<double> E;  // [nsa * nsb][nh] ket form, no contraction coefficients
+    std::vector<double> E;  // [nh][nsa * nsb] ket form, no contraction coefficients
     std::vector<double> w;  // per column pair: ca_i * cb_j, 0 where that pair screens the primitive out
 };
 
@@ -585,6 +616,7 @@ const std::vector<std::pair<int, int>>& ket_pairs(const f2ints_basis* cbs) {
 #pragma omp parallel num_threads(nthreads())
         {
             PrimE pe;
+            std::vector<double> Etmp;
 #pragma omp for schedule(dynamic)
             for (size_t k = 0; k < fps.size(); ++k) {
                 const int fa = fps[k].first, fb = fps[k].second;
@@ -622,7 +654,13 @@ const std::vector<std::pair<int, int>>& ket_pairs(const f2ints_basis* cbs) {
                         if (!any) continue;
                         pp.p = p;
                         for (int d = 0; d < 3; ++d) pp.P[d] = (a * A.A[d] + b * B.A[d]) / p;
-                        pe.compute(A, B, a, b, true, pp.E);
+                        // stored h-major so that the aux contraction runs over the
+                        // pair components in the inner loop
+                        pe.compute(A, B, a, b, true, Etmp);
+                        const int nab = nsph(A.l) * nsph(B.l), nh = nherm(A.l + B.l);
+                        pp.E.resize(Etmp.size());
+                        for (int s2 = 0; s2 < nab; ++s2)
+                            for (int h = 0; h < nh; ++h) pp.E[(size_t)h * nab + s2] = Etmp[(size_t)s2 * nh + h];
                         f->prims.push_back(std::move(pp));
                     }
                 f->q.assign(ncp, -1);
@@ -664,6 +702,32 @@ struct Scratch3c {
     }
 };
 
+// Positions in hidx().list of R_{c + h} for the Cartesian component c of an
+// aux shell of angular momentum lP and the Hermite index h of a pair of total
+// angular momentum lab: [c * nherm(lab) + h].
+const std::vector<int>& y_index(int lP, int lab) {
+    static const std::vector<std::vector<std::vector<int>>> tab = [] {
+        const Harmonics& Hm = harm();
+        const HermIndex& H = hidx();
+        std::vector<std::vector<std::vector<int>>> t(LMAX + 1, std::vector<std::vector<int>>(2 * LMAX + 1));
+        for (int l = 0; l <= LMAX; ++l)
+            for (int L = 0; L <= 2 * LMAX; ++L) {
+                const int nh = nherm(L);
+                std::vector<int>& v = t[l][L];
+                v.resize((size_t)ncart(l) * nh);
+                for (int c = 0; c < ncart(l); ++c) {
+                    const CartComp& cp = Hm.cart[l][c];
+                    for (int h = 0; h < nh; ++h) {
+                        const auto& tuv = H.list[L][h];
+                        v[(size_t)c * nh + h] = H.at(cp.x + tuv[0], cp.y + tuv[1], cp.z + tuv[2]);
+                    }
+                }
+            }
+        return t;
+    }();
+    return tab[lP][lab];
+}
+
 // (P|ab) of aux shell P for every column pair c of family pair f, into
 // w.acc[(c * nsP + sP) * nab + sa * nsb + sb].  Per column pair the terms are
 // added in the order of a single-shell evaluation (aux primitive, then
@@ -678,6 +742,7 @@ void eval_fam(const f2ints_basis* bs, const FamPair& f, const Shell& P, Scratch3
     const int ncp = (int)f.cols.size();
     w.acc.assign((size_t)ncp * nsP * nab, 0.0);
     static const double two_pi52 = 2.0 * std::pow(PI, 2.5);
+    const std::vector<int>& yix = y_index(lP, lab);
     for (size_t k = 0; k < P.alpha.size(); ++k) {
         const double g = P.alpha[k];
         const double gl = std::pow(2.0 * g, -lP);  // once per aux primitive
@@ -690,22 +755,21 @@ void eval_fam(const f2ints_basis* bs, const FamPair& f, const Shell& P, Scratch3
                 double* y = &w.Y[(size_t)sP * nhab];
                 std::fill(y, y + nhab, 0.0);
                 for (const SphTerm& tp : Hm.sparse[lP][sP]) {
-                    const CartComp& cp = Hm.cart[lP][tp.cart];
-                    for (int h = 0; h < nhab; ++h) {
-                        const auto& tuv = H.list[lab][h];
-                        y[h] += tp.c * w.R[H.at(cp.x + tuv[0], cp.y + tuv[1], cp.z + tuv[2])];
-                    }
+                    const int* ix = &yix[(size_t)tp.cart * nhab];
+                    for (int h = 0; h < nhab; ++h) y[h] += tp.c * w.R[ix[h]];
                 }
             }
-            // V[sP][s2] = sum_h E[s2][h] Y[sP][h], shared by the column pairs
+            // V[sP][s2] = sum_h E[s2][h] Y[sP][h], shared by the column pairs;
+            // each V[sP][s2] sums over h in ascending order, the inner loop
+            // runs over independent s2
             for (int sP = 0; sP < nsP; ++sP) {
                 const double* y = &w.Y[(size_t)sP * nhab];
                 double* v = &w.V[(size_t)sP * nab];
-                for (int s2 = 0; s2 < nab; ++s2) {
-                    const double* e = &pp.E[(size_t)s2 * nhab];
-                    double t = 0.0;
-                    for (int h = 0; h < nhab; ++h) t += e[h] * y[h];
-                    v[s2] = t;
+                std::fill(v, v + nab, 0.0);
+                for (int h = 0; h < nhab; ++h) {
+                    const double yh = y[h];
+                    const double* e = &pp.E[(size_t)h * nab];
+                    for (int s2 = 0; s2 < nab; ++s2) v[s2] += e[s2] * yh;
                 }
             }
             for (int c = 0; c < ncp; ++c) {
