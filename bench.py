@@ -7,12 +7,14 @@ copy) and the occupied coefficients are synthetic and already resident in HBM
 when the timed region starts.  One step = one RHF J/K build (D = C C^T, J, K).
 
 The headline value is the builds/s of the precision form (``--mixed``,
-default fp32); the FP64 plain form is timed on the same inputs, and the line
+default bf16x3); the FP64 plain form is timed on the same inputs, and the line
 reports the speed-up and the max-abs J/K difference between the two.
 
 Printed: one JSON line with metric/value/unit, the roofline of the dominant
-kernel (K half-transform, measured with HIP events on the launch stream) and
-the CPU baseline (the numpy oracle on a bounded aux-slice sample, scaled).
+kernel (K half-transform, measured with HIP events on the launch stream), the
+achievable MFMA and HBM-read ceilings measured on the device by peak probes
+beside the published ones, and the CPU baseline (the numpy oracle on a bounded
+aux-slice sample, scaled).
 """
 
 import argparse
@@ -51,6 +53,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 4 GiB device copy (x3), the reference for HBM byte counters")
+    ap.add_argument("--peaks", action="store_true",
+                    help="run only the MFMA and HBM-read peak probes and print their JSON")
     ap.add_argument("--otf", type=int, default=None, metavar="N",
                     help="time the integral-direct builder on (H2O)_N instead (cc-pVDZ/cc-pVTZ-JKFIT)")
     ap.add_argument("--otf-mem", type=float, default=4000.0, help="jk_mem_thres_mb of the integral-direct builder")
@@ -84,6 +88,45 @@ def copy_calibration():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_per_launch": 4 << 30, "launches": 3}))
+
+
+def measured_peaks(read_buf=None, read_reps=3):
+    """Achievable ceilings of this device from libf2probe.so: TFLOP/s of
+    register-only MFMA chains (bf16 16x16x32, the BF16x3 kernels' instruction;
+    fp32 16x16x4; fp64 16x16x4), eight waves per SIMD, ~20 ms per launch, and
+    the HBM GB/s of one streaming read of read_buf (default: a fresh 16 GiB
+    buffer).  Timed with HIP events on the current stream."""
+    import ctypes
+
+    import torch
+
+    from forte2_amd import _native as nat
+
+    lib = nat.probe_lib()
+    n = ctypes.c_int64()
+    nat.probe_check(lib.f2probe_sink_len(ctypes.byref(n)), "f2probe_sink_len")
+    sink = torch.empty(n.value, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for kind, iters in (("bf16", 100000), ("fp32", 50000), ("fp64", 25000)):
+        fl, ms = ctypes.c_double(), ctypes.c_double()
+        nat.probe_check(lib.f2probe_mfma(nat.F2PROBE_KIND[kind], iters, ctypes.c_void_p(sink.data_ptr()), n.value,
+                                         ctypes.c_void_p(st), ctypes.byref(fl), ctypes.byref(ms)), "f2probe_mfma")
+        out[f"mfma_{kind}_TFLOPs"] = fl.value / (ms.value * 1e-3) / 1e12
+        out[f"mfma_{kind}_ms_per_launch"] = ms.value
+    own = read_buf is None
+    if own:
+        read_buf = torch.empty((16 << 30) // 4, dtype=torch.float32, device="cuda")
+    nbytes = read_buf.numel() * read_buf.element_size()
+    ms = ctypes.c_double()
+    nat.probe_check(lib.f2probe_read(ctypes.c_void_p(read_buf.data_ptr()), nbytes, read_reps,
+                                     ctypes.c_void_p(sink.data_ptr()), n.value, ctypes.c_void_p(st),
+                                     ctypes.byref(ms)), "f2probe_read")
+    out["hbm_read_GBps"] = nbytes / (ms.value * 1e-3) / 1e9
+    out["hbm_read_bytes"] = nbytes
+    if own:
+        del read_buf
+    return out
 
 
 def scf_mode(args):
@@ -356,6 +399,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.peaks:
+        print(json.dumps({"measured_peaks": measured_peaks()}))
+        return
     import torch
 
     from forte2_amd.jkbuilder import FockBuilder
@@ -397,6 +443,10 @@ def main():
     j_ms = ph_mix["j_gamma"] + ph_mix["j_acc"]
     j64_ms = ph64["j_gamma"] + ph64["j_acc"]
 
+    peaks = measured_peaks(read_buf=fb.B_fp32())
+    mpeak = peaks["mfma_bf16_TFLOPs" if args.mixed == "bf16x3" else "mfma_fp32_TFLOPs"]
+    print(f"[bench] measured peaks: {json.dumps(peaks)}", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_sample_aux > 0:
         cpu = cpu_baseline(B, C, min(args.cpu_sample_aux, Q))
@@ -429,6 +479,7 @@ def main():
         "J": {
             "GBps": jb_mix / (j_ms * 1e-3) / 1e9, "fp64_GBps": jb64 / (j64_ms * 1e-3) / 1e9,
             "frac_of_hbm_peak": jb_mix / (j_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "frac_of_measured_read": jb_mix / (j_ms * 1e-3) / 1e9 / peaks["hbm_read_GBps"],
             "bytes_algorithmic": jb_mix, "layout": "lower triangle read from full-square B",
         },
         "K": {
@@ -453,7 +504,12 @@ def main():
             # BF16x3 runs three bf16 MFMA products per algorithmic product
             "executed_flops_per_algorithmic": 3 if args.mixed == "bf16x3" else 1,
             "executed_frac": achieved * (3 if args.mixed == "bf16x3" else 1) / peak,
+            # the same executed rate against what register-only chains of the
+            # kernel's MFMA instruction reach on this device (measured_peaks)
+            "measured_peak": mpeak,
+            "executed_frac_of_measured": achieved * (3 if args.mixed == "bf16x3" else 1) / mpeak,
         },
+        "measured_peaks": peaks,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

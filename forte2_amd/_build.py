@@ -2,6 +2,8 @@
 
 * ``libf2jk.so``  -- HIP J/K kernels + C ABI (``include/f2jk.h``), gfx950 only.
 * ``libf2ints.so`` -- host integral engine (``include/f2ints.h``), g++/OpenMP.
+* ``libf2probe.so`` -- MFMA and HBM peak probes (``include/f2probe.h``), used
+  by the benchmark only.
 
 Both are plain shared objects loaded with ctypes; nothing is JIT-compiled at
 import time.  ``python -m forte2_amd._build`` rebuilds both.
@@ -61,6 +63,17 @@ def build_jk():
     return out
 
 
+def build_probe():
+    out = os.path.join(PKG, "libf2probe.so")
+    src = os.path.join(CSRC, "probe.hip")
+    tmp = out + ".tmp"
+    # accumulators in VGPRs: in the AGPR form the compiler copies the loop-carried
+    # accumulators between the register files on every round of the probe loop
+    _run([_hipcc(), *HIPCC_FLAGS, "-mllvm", "-amdgpu-mfma-vgpr-form", f"-I{INCLUDE}", src, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_ints():
     out = os.path.join(PKG, "libf2ints.so")
     srcdir = os.path.join(CSRC, "ints")
@@ -74,7 +87,7 @@ def build_ints():
 
 
 def build_all():
-    return [p for p in (build_ints(), build_jk()) if p]
+    return [p for p in (build_ints(), build_jk(), build_probe()) if p]
 
 
 if __name__ == "__main__":

@@ -1,4 +1,5 @@
-"""ctypes bindings of the native libraries (``include/f2jk.h``, ``include/f2ints.h``).
+"""ctypes bindings of the native libraries (``include/f2jk.h``, ``include/f2ints.h``,
+``include/f2probe.h``).
 
 Nothing here falls back to Python: if a library is missing the loader raises,
 so a GPU path can never silently run on the host.
@@ -125,3 +126,32 @@ def ints_lib():
     _ints_proto.declare(lib)
     _ints = lib
     return lib
+
+
+F2PROBE_SYMBOLS = ("f2probe_sink_len", "f2probe_mfma", "f2probe_read", "f2probe_last_error")
+F2PROBE_KIND = {"bf16": 0, "fp32": 1, "fp64": 2}
+_probe = None
+
+
+def probe_lib():
+    """Load libf2probe.so (MFMA / HBM peak probes, ``include/f2probe.h``)."""
+    global _probe
+    if _probe is not None:
+        return _probe
+    lib = _load("libf2probe.so")
+    V, I, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    lib.f2probe_sink_len.argtypes = [c_int64_p]
+    lib.f2probe_mfma.argtypes = [I, I, V, I64, V, c_double_p, c_double_p]
+    lib.f2probe_read.argtypes = [V, I64, I, V, I64, V, c_double_p]
+    lib.f2probe_last_error.argtypes = []
+    lib.f2probe_last_error.restype = ctypes.c_char_p
+    for s in F2PROBE_SYMBOLS[:-1]:
+        getattr(lib, s).restype = I
+    _probe = lib
+    return lib
+
+
+def probe_check(rc, what=""):
+    if rc != 0:
+        msg = probe_lib().f2probe_last_error().decode()
+        raise (ValueError if rc == -1 else NativeError)(f"{what} failed ({rc}): {msg}")

@@ -21,5 +21,5 @@ def test_help_lists_the_modes():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode == 0
-    for flag in ("--steps", "--warmup", "--gpus", "--serve", "--otf", "--scf"):
+    for flag in ("--steps", "--warmup", "--gpus", "--serve", "--otf", "--scf", "--peaks"):
         assert flag in p.stdout

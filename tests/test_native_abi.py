@@ -34,3 +34,18 @@ def test_f2jk_errors_without_gpu_are_reported():
     assert b"null context" in lib.f2jk_last_error()
     rc = lib.f2jk_ctx_destroy(None)
     assert rc == 0
+
+
+def test_f2probe_exports_every_declared_symbol():
+    from forte2_amd import _native
+
+    lib = _native.probe_lib()
+    names = _declared("f2probe.h")
+    assert set(names) == set(_native.F2PROBE_SYMBOLS)
+    for n in names:
+        assert hasattr(lib, n), n
+    # argument checks run before any device call
+    ms = ctypes.c_double()
+    assert lib.f2probe_read(None, 16, 1, None, 0, None, ctypes.byref(ms)) == -1
+    assert b"null argument" in lib.f2probe_last_error()
+
