@@ -168,6 +168,7 @@ class _SCF:
         self.H = torch.as_tensor(sysm.ints_hcore(), dtype=f64).to(dev)
         self.Vnn = float(sysm.nuclear_repulsion)
         self.X = canonical_orth(self.S, getattr(sysm, "overlap_ortho_rtol", 1e-8))
+        self._XT = None
         self.nbf = sysm.nbf
         self.nmo = int(self.X.shape[1])
         self.naux = sysm.naux
@@ -175,6 +176,16 @@ class _SCF:
     def _eigh(self, F):
         e, c = torch.linalg.eigh(self.X.T @ F @ self.X)
         return e, self.X @ c
+
+    def _eigh_batch(self, F):
+        """eigh of X^T F_s X for a (nspin, N, N) stack in one batched call: the
+        library's eigensolver is latency-bound at these sizes, so the pair
+        costs about one single call (both operands of every product are
+        contiguous)."""
+        if self._XT is None:
+            self._XT = self.X.T.contiguous()
+        e, c = torch.linalg.eigh(torch.matmul(self._XT, torch.matmul(F, self.X)))
+        return e, torch.matmul(self.X, c)
 
     def _guess(self):
         if self.guess_type == "minao":
@@ -398,9 +409,9 @@ class UHF(_SCF):
         return [flat[: N * N].reshape(N, N), flat[N * N:].reshape(N, N)]
 
     def _diagonalize(self, F):
-        ea, ca = self._eigh(F[0])
-        eb, cb = self._eigh(F[1])
-        return [ea, eb], [ca, cb]
+        # both spins in one batched eigh (scf_base.py:113-116 per spin)
+        e, c = self._eigh_batch(torch.stack(F))
+        return [e[0], e[1]], [c[0], c[1]]
 
     def _spin(self):
         a, b = self._occ()
