@@ -291,7 +291,9 @@ class _SCF:
             if self.log:
                 self.log(f"{self.iter + 1:4d} {self.E:20.12f} {dE:12.4e} {dD:12.4e} {stage:>5s} {diis.status}")
             if stage == "fp64" and abs(dE) < self.e_tol and dD < self.d_tol:
-                self.eps, self.C = self._diagonalize(Fx)
+                # final iteration: diagonalise the new, non-extrapolated Fock
+                # matrix (scf_base.py:214-221; F_canon was rebuilt at :196)
+                self.eps, self.C = self._diagonalize(F)
                 self.D = self._density()
                 F = yield from self._fock_gen("fp64")
                 self.E = self.Vnn + self._energy(F)

@@ -128,7 +128,9 @@ def run_rhf(S, H, Vnn, jk, nel, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxi
         if trace is not None:
             trace.append((E, dE, dD))
         if abs(dE) < e_tol and dD < d_tol:
-            eps, C = _eigh(X, Fx)
+            # the final iteration diagonalises the new, non-extrapolated Fock
+            # matrix (scf_base.py:214-221: F_canon was rebuilt at :196)
+            eps, C = _eigh(X, F)
             D = C[:, :na] @ C[:, :na].T
             F = fock(C)
             E = Vnn + np.sum(D * (H + F))
@@ -189,8 +191,8 @@ def run_uhf(S, H, Vnn, jk, nel, ms, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, 
         dE = E - Eold
         dD = sum(np.linalg.norm(d - do) for d, do in zip(D, Dold))
         if abs(dE) < e_tol and dD < d_tol:
-            ea, Ca = _eigh(X, Fx[0])
-            eb, Cb = _eigh(X, Fx[1])
+            ea, Ca = _eigh(X, F[0])  # the new Fock matrices, as in RHF above
+            eb, Cb = _eigh(X, F[1])
             C = [Ca, Cb]
             D = dens(C)
             F = fock(C)

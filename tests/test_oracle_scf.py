@@ -44,6 +44,17 @@ def test_rhf_golden(xyz, basis, aux, unit, eref):
     assert r.E == pytest.approx(eref, abs=5e-8)
 
 
+def test_final_iteration_moves_to_the_new_fock():
+    # scf_base.py:214-221: after convergence the new, non-extrapolated Fock
+    # matrix is diagonalised once more and rebuilt.  Diagonalising the DIIS
+    # matrix of the last step again would reproduce its density, and the
+    # final energy would equal the last loop energy bit for bit.
+    trace = []
+    r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", e_tol=1e-10, trace=trace)
+    assert r.E != trace[-1][0]
+    assert abs(r.E - trace[-1][0]) < 1e-10
+
+
 def test_rhf_hcore_guess_same_state():
     r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", guess="hcore", e_tol=1e-10)
     assert r.E == pytest.approx(-76.02176598836786, abs=5e-8)

@@ -57,6 +57,9 @@ def test_plain_form_matches_oracle(water):
                        X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
     assert abs(r.E - ref.E) < 1e-10
     assert r.iter == ref.iter
+    # the final iteration diagonalises the new Fock matrix, which moves the
+    # density once more (scf_base.py:214-221)
+    assert r.E != r.history[-1][2] and abs(r.E - r.history[-1][2]) < 1e-10
 
 
 def test_precision_schedule_and_requests(water):
