@@ -13,7 +13,7 @@ import sys
 
 SETS = ["sto-3g", "cc-pvdz", "cc-pvtz", "cc-pvqz", "cc-pvtz-jkfit", "cc-pvqz-jkfit",
         "def2-universal-jkfit", "sap_helfem_large"]
-ELEMENTS = ["1", "3", "6", "7", "8", "9", "10"]  # H Li C N O F Ne
+ELEMENTS = ["1", "3", "6", "7", "8", "9", "10", "17"]  # H Li C N O F Ne Cl
 
 
 def main(src):

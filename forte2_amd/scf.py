@@ -128,8 +128,9 @@ class _SCF:
     def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
                  d_tol=1e-6, maxiter=100, guess_type="minao", die_if_not_converged=True,
                  precision="fp64", switch_inc=1e-4, switch_fp64=1e-6, incremental=True,
-                 device=None, log=None):
+                 device=None, log=None, level_shift=None, level_shift_thresh=1e-5):
         self.charge = charge
+        self.level_shift, self.level_shift_thresh = level_shift, level_shift_thresh
         self.do_diis, self.diis_start, self.diis_nvec, self.diis_min = do_diis, diis_start, diis_nvec, diis_min
         self.e_tol, self.d_tol, self.maxiter = e_tol, d_tol, maxiter
         self.guess_type = guess_type
@@ -151,7 +152,36 @@ class _SCF:
             raise AssertionError("Number of electrons must be non-negative.")
         self._parse_state()
         self.C = None
+        self._validate_level_shift()
         return self
+
+    def _validate_level_shift(self):
+        """The checks of scf_base.py:101-111: non-negative; a pair only for
+        UHF, where one number is used for both spins."""
+        ls = self.level_shift
+        if ls is None:
+            return
+        if isinstance(ls, tuple):
+            if self.method != "UHF":
+                raise ValueError("Tuple level_shift is only valid for UHF.")
+            if len(ls) != 2:
+                raise ValueError("Tuple level_shift must have length 2 for UHF.")
+            return
+        if ls < 0.0:
+            raise ValueError("level_shift must be non-negative.")
+        if self.method == "UHF":
+            self.level_shift = (float(ls), float(ls))
+
+    def _apply_level_shift(self, F):
+        """F_s + shift_s (S - S D_s S) on the current densities (rhf.py:90-95,
+        uhf.py:150-159); shifts under 1e-4 leave F as it is."""
+        cur = self._current_level_shift
+        if cur is None:
+            return F
+        shifts = cur if isinstance(cur, tuple) else (cur,) * len(F)
+        if all(x < 1e-4 for x in shifts):
+            return F
+        return [f + x * (self.S - self.S @ d @ self.S) for x, f, d in zip(shifts, F, self.D)]
 
     # ------------------------------------------------------------------ setup
     def _setup(self):
@@ -262,9 +292,10 @@ class _SCF:
         self.iter += 1
         self.history.append((self.iter, stage, self.E))
         self.lp_error_vs_fp64 = None
+        self._current_level_shift = self.level_shift
         for _ in range(self.maxiter):
             grad = self._ao_grad(F)
-            Fx = self._diis_update(diis, F, grad)
+            Fx = self._apply_level_shift(self._diis_update(diis, F, grad))
             self.eps, self.C = self._diagonalize(Fx)
             self.D = self._density()
             # stage transitions of the precision form (decided on the previous dE)
@@ -286,6 +317,8 @@ class _SCF:
             stage = next_stage
             dE = self.E - Eold
             self._last_dE = dE
+            if abs(dE) < self.level_shift_thresh:  # scf_base.py:203-204
+                self._current_level_shift = None
             dD = sum(float(torch.linalg.norm(d - do)) for d, do in zip(self.D, Dold))
             self.history.append((self.iter + 1, stage, self.E))
             if self.log:

@@ -97,7 +97,7 @@ class SCFResult:
 
 
 def run_rhf(S, H, Vnn, jk, nel, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxiter=100,
-            diis=(1, 8, 2), do_diis=True, C0=None, trace=None):
+            diis=(1, 8, 2), do_diis=True, C0=None, trace=None, level_shift=None, level_shift_thresh=1e-5):
     """RHF (scf_base.py:121-251 with rhf.py:35-65): returns E, C, eps, D, iterations."""
     if nel % 2:
         raise AssertionError("RHF requires an even number of electrons.")
@@ -116,14 +116,19 @@ def run_rhf(S, H, Vnn, jk, nel, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxi
     Eold, Dold = E, D
     it = 1
     converged = False
+    shift = level_shift  # scf_base.py:131; switched off below at :203-204
     for _ in range(maxiter):
         grad = X.T @ (F @ D @ S - S @ D @ F) @ X
         Fx = dd.update(F, grad)
+        if shift is not None and shift >= 1e-4:  # rhf.py:90-95
+            Fx = Fx + shift * (S - S @ D @ S)
         eps, C = _eigh(X, Fx)
         D = C[:, :na] @ C[:, :na].T
         F = fock(C)
         E = Vnn + np.sum(D * (H + F))
         dE = E - Eold
+        if abs(dE) < level_shift_thresh:
+            shift = None
         dD = np.linalg.norm(D - Dold)
         if trace is not None:
             trace.append((E, dE, dD))
@@ -146,7 +151,7 @@ def run_rhf(S, H, Vnn, jk, nel, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxi
 
 
 def run_uhf(S, H, Vnn, jk, nel, ms, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxiter=100,
-            diis=(1, 8, 2), do_diis=True):
+            diis=(1, 8, 2), do_diis=True, level_shift=None, level_shift_thresh=1e-5):
     """UHF (scf_base.py:121-251 with uhf.py:40-127)."""
     twicems = int(round(2 * ms))
     if nel % 2 != twicems % 2:
@@ -178,10 +183,14 @@ def run_uhf(S, H, Vnn, jk, nel, ms, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, 
     Eold, Dold = E, D
     it = 1
     converged = False
+    shift = None if level_shift is None else (
+        tuple(level_shift) if isinstance(level_shift, tuple) else (level_shift, level_shift))
     for _ in range(maxiter):
         grad = np.hstack([(X.T @ (f @ d @ S - S @ d @ f) @ X).ravel() for d, f in zip(D, F)])
         flat = dd.update(np.hstack([f.ravel() for f in F]), grad)
         Fx = [flat[: N * N].reshape(N, N), flat[N * N:].reshape(N, N)]
+        if shift is not None and not all(x < 1e-4 for x in shift):  # uhf.py:150-159
+            Fx = [f + x * (S - S @ d @ S) for x, f, d in zip(shift, Fx, D)]
         ea, Ca = _eigh(X, Fx[0])
         eb, Cb = _eigh(X, Fx[1])
         C = [Ca, Cb]
@@ -189,6 +198,8 @@ def run_uhf(S, H, Vnn, jk, nel, ms, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, 
         F = fock(C)
         E = Vnn + energy(D, F)
         dE = E - Eold
+        if abs(dE) < level_shift_thresh:
+            shift = None
         dD = sum(np.linalg.norm(d - do) for d, do in zip(D, Dold))
         if abs(dE) < e_tol and dD < d_tol:
             ea, Ca = _eigh(X, F[0])  # the new Fock matrices, as in RHF above

@@ -55,6 +55,32 @@ def test_final_iteration_moves_to_the_new_fock():
     assert abs(r.E - trace[-1][0]) < 1e-10
 
 
+FORMAMIDE = """C   0.000000  0.418626 0.000000
+H  -0.460595  1.426053 0.000000
+O   1.196516  0.242075 0.000000
+N  -0.936579 -0.568753 0.000000
+H  -0.634414 -1.530889 0.000000
+H  -1.921071 -0.362247 0.000000"""
+
+
+def test_rhf_level_shift_golden():
+    # tests/scf/test_rhf.py:62-79: level shift 0.5 without DIIS
+    r = _run(FORMAMIDE, "cc-pvtz", "cc-pvtz-jkfit", level_shift=0.5, do_diis=False)
+    assert r.E == pytest.approx(-169.001135897278, abs=5e-8)
+    # the reference's note (test_rhf.py:64-66): ~30 iterations with the
+    # shift, ~60 without it
+    plain = _run(FORMAMIDE, "cc-pvtz", "cc-pvtz-jkfit", do_diis=False)
+    assert r.iter < 40 < 50 < plain.iter
+    assert plain.E == pytest.approx(r.E, abs=5e-8)
+
+
+def test_cl2_golden():
+    # tests/scf/test_rhf.py:97-117 (run there through the integral-direct
+    # builder; the DF energy is the same in core)
+    r = _run("Cl 0 0 0\nCl 0 0 1.681", "cc-pvqz", "cc-pvqz-jkfit", e_tol=1e-10)
+    assert r.E == pytest.approx(-918.943349338796, abs=5e-8)
+
+
 def test_rhf_hcore_guess_same_state():
     r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", guess="hcore", e_tol=1e-10)
     assert r.E == pytest.approx(-76.02176598836786, abs=5e-8)

@@ -148,6 +148,20 @@ def test_rhf_through_factory(dev):
     assert scf.E == pytest.approx(-76.02176598836786, abs=5e-8)
 
 
+def test_cl2_golden_through_otf(dev):
+    # tests/scf/test_rhf.py:97-117: Cl2 cc-pVQZ / cc-pVQZ-JKFIT with the
+    # integral-direct builder assigned to the system at a 10 MB budget
+    from forte2_amd.scf import RHF
+
+    print("[otf-gpu] Cl2 cc-pVQZ / cc-pVQZ-JKFIT, 10 MB", flush=True)
+    s = System("Cl 0 0 0\nCl 0 0 1.681", "cc-pvqz", "cc-pvqz-jkfit", device=dev)
+    s.fock_builder = FockBuilderOTF(s, jk_mem_thres_mb=10, backend="libcint", device=dev)
+    assert len(s.fock_builder.blocks) > 1
+    scf = RHF(charge=0)(s).run()
+    print(f"   E = {scf.E:.12f}", flush=True)
+    assert scf.E == pytest.approx(-918.943349338796, abs=5e-8)
+
+
 def test_errors(dev):
     s = System(N2, "cc-pvdz", "cc-pvtz-jkfit", unit="bohr", device=dev)
     with pytest.raises(ValueError, match="too low"):

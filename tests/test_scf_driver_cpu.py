@@ -62,6 +62,37 @@ def test_plain_form_matches_oracle(water):
     assert r.E != r.history[-1][2] and abs(r.E - r.history[-1][2]) < 1e-10
 
 
+@pytest.mark.parametrize("method", ["rhf", "uhf"])
+def test_level_shift_matches_oracle(water, method):
+    # rhf.py:90-95, uhf.py:150-159, switched off once |dE| < 1e-5
+    # (scf_base.py:203-204): same energy and iteration count as the oracle
+    s, B = water
+    s.fock_builder = HostBuilder(B)
+    common = dict(X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10, do_diis=False)
+    if method == "rhf":
+        r = RHF(charge=0, e_tol=1e-10, do_diis=False, level_shift=0.3)(s).run()
+        ref = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                           level_shift=0.3, **common)
+    else:
+        r = UHF(charge=1, ms=0.5, e_tol=1e-10, do_diis=False, level_shift=(0.2, 0.4))(s).run()
+        ref = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - 1,
+                           0.5, level_shift=(0.2, 0.4), **common)
+    assert abs(r.E - ref.E) < 1e-10
+    assert r.iter == ref.iter
+
+
+def test_level_shift_validation(water):
+    s, _ = water
+    with pytest.raises(ValueError, match="non-negative"):
+        RHF(charge=0, level_shift=-0.1)(s)
+    with pytest.raises(ValueError, match="only valid for UHF"):
+        RHF(charge=0, level_shift=(0.1, 0.2))(s)
+    with pytest.raises(ValueError, match="length 2"):
+        UHF(charge=0, ms=0.0, level_shift=(0.1, 0.2, 0.3))(s)
+    u = UHF(charge=0, ms=0.0, level_shift=0.25)(s)
+    assert u.level_shift == (0.25, 0.25)
+
+
 def test_precision_schedule_and_requests(water):
     s, B = water
     hb = HostBuilder(B)

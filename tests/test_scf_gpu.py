@@ -78,6 +78,18 @@ def test_rhf_h2o16_plain_form(dev):
     assert abs(scf.E - ref.E) <= 1e-10
 
 
+def test_rhf_level_shift_golden(dev):
+    # tests/scf/test_rhf.py:62-79: formamide, level shift 0.5, no DIIS
+    print("[scf-gpu] formamide cc-pVTZ/cc-pVTZ-JKFIT RHF, level shift 0.5", flush=True)
+    xyz = ("C 0.000000 0.418626 0.000000\nH -0.460595 1.426053 0.000000\nO 1.196516 0.242075 0.000000\n"
+           "N -0.936579 -0.568753 0.000000\nH -0.634414 -1.530889 0.000000\nH -1.921071 -0.362247 0.000000")
+    s = System(xyz, "cc-pvtz", "cc-pvtz-jkfit", device=dev)
+    scf = RHF(charge=0, level_shift=0.5, do_diis=False)(s).run()
+    print(f"   E = {scf.E:.12f} iters {scf.iter}", flush=True)
+    assert scf.E == pytest.approx(-169.001135897278, abs=5e-8)
+    assert scf.iter < 40
+
+
 def test_uhf_water_triplet_golden(dev):
     s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit", device=dev)
     scf = UHF(charge=0, ms=1.0, e_tol=1e-10)(s).run()
