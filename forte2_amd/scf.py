@@ -123,6 +123,21 @@ def _procrustes_split(Cn, Co, S):
     return s * P + A / s, s * P - A / s
 
 
+def guess_mix(C, homo, angle=np.pi / 4):
+    """Break the alpha/beta symmetry of an ms = 0 guess (scf_utils.py:76-104):
+    the HOMO and LUMO columns are rotated by +angle for alpha and -angle for
+    beta (Szabo and Ostlund 3.8.7)."""
+    c, s = float(np.cos(angle)), float(np.sin(angle))
+    out = []
+    for sg in (1.0, -1.0):
+        M = C.clone()
+        h, l = C[:, homo], C[:, homo + 1]
+        M[:, homo] = c * h + sg * s * l
+        M[:, homo + 1] = -sg * s * h + c * l
+        out.append(M)
+    return out
+
+
 # ---------------------------------------------------------------- SCF base
 class _SCF:
     def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
@@ -409,11 +424,14 @@ class UHF(_SCF):
         if self.nel % 2 != tw % 2:
             raise ValueError(f"{self.nel} electrons is incompatible with ms={self.ms}!")
         self.na, self.nb = (self.nel + tw) // 2, (self.nel - tw) // 2
+        self.twicems = tw
         if self.na < 0 or self.nb < 0:
             raise AssertionError("UHF requires non-negative numbers of alpha and beta electrons.")
 
     def _guess_orbitals(self):
         C = self._guess()
+        if self.twicems == 0 and self.guess_mix:
+            return guess_mix(C, self.nel // 2 - 1)
         return [C, C.clone()]
 
     def _occ(self):

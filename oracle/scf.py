@@ -151,7 +151,7 @@ def run_rhf(S, H, Vnn, jk, nel, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxi
 
 
 def run_uhf(S, H, Vnn, jk, nel, ms, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, maxiter=100,
-            diis=(1, 8, 2), do_diis=True, level_shift=None, level_shift_thresh=1e-5):
+            diis=(1, 8, 2), do_diis=True, level_shift=None, level_shift_thresh=1e-5, guess_mix=False):
     """UHF (scf_base.py:121-251 with uhf.py:40-127)."""
     twicems = int(round(2 * ms))
     if nel % 2 != twicems % 2:
@@ -162,6 +162,17 @@ def run_uhf(S, H, Vnn, jk, nel, ms, X=None, V_sap=None, e_tol=1e-9, d_tol=1e-6, 
     dd = DIIS(*diis, do_diis=do_diis)
     C0 = guess(X, H, V_sap)
     C = [C0, C0]
+    if twicems == 0 and guess_mix:
+        # uhf.py:85-86 and scf_utils.py:76-104: HOMO/LUMO Givens rotation by
+        # +pi/4 (alpha) and -pi/4 (beta)
+        h = nel // 2 - 1
+        q = np.pi / 4
+        C = []
+        for sq in (np.sin(q), -np.sin(q)):
+            M = C0.copy()
+            M[:, h] = np.cos(q) * C0[:, h] + sq * C0[:, h + 1]
+            M[:, h + 1] = -sq * C0[:, h] + np.cos(q) * C0[:, h + 1]
+            C.append(M)
 
     def dens(C):
         return [C[0][:, :na] @ C[0][:, :na].T, C[1][:, :nb] @ C[1][:, :nb].T]

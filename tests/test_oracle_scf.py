@@ -81,6 +81,19 @@ def test_cl2_golden():
     assert r.E == pytest.approx(-918.943349338796, abs=5e-8)
 
 
+def test_coulson_fischer_guess_mix():
+    # tests/scf/test_uhf.py:73-90: stretched H2, RHF and the symmetry-broken
+    # ms = 0 UHF from the mixed guess (without the mix UHF stays on RHF)
+    h2 = "H 0 0 0\nH 0 0 2.7"
+    r = _run(h2, "cc-pvtz", "cc-pvqz-jkfit")
+    assert r.E == pytest.approx(-0.852767190652, abs=5e-8)
+    u = _run(h2, "cc-pvtz", "cc-pvqz-jkfit", ms=0.0, guess_mix=True)
+    assert u.E == pytest.approx(-1.000027890395, abs=5e-8)
+    assert u.S2 == pytest.approx(0.9874559102128492, abs=5e-8)
+    plain = _run(h2, "cc-pvtz", "cc-pvqz-jkfit", ms=0.0)
+    assert plain.E == pytest.approx(r.E, abs=5e-8)
+
+
 def test_rhf_hcore_guess_same_state():
     r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", guess="hcore", e_tol=1e-10)
     assert r.E == pytest.approx(-76.02176598836786, abs=5e-8)

@@ -81,6 +81,20 @@ def test_level_shift_matches_oracle(water, method):
     assert r.iter == ref.iter
 
 
+def test_guess_mix_matches_oracle():
+    # the device driver's mixed ms = 0 guess (scf_utils.py:76-104) on the
+    # reference's stretched H2 (tests/scf/test_uhf.py:73-90)
+    s = System("H 0 0 0\nH 0 0 2.7", "cc-pvtz", "cc-pvqz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    s.fock_builder = HostBuilder(B)
+    u = UHF(charge=0, ms=0.0, guess_mix=True, e_tol=1e-10)(s).run()
+    ref = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum, 0.0,
+                       X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10, guess_mix=True)
+    assert abs(u.E - ref.E) < 1e-10 and u.iter == ref.iter
+    assert u.E == pytest.approx(-1.000027890395, abs=5e-8)
+    assert u.S2 == pytest.approx(0.9874559102128492, abs=5e-8)
+
+
 def test_level_shift_validation(water):
     s, _ = water
     with pytest.raises(ValueError, match="non-negative"):

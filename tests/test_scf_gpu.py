@@ -90,6 +90,14 @@ def test_rhf_level_shift_golden(dev):
     assert scf.iter < 40
 
 
+def test_uhf_guess_mix_golden(dev):
+    # tests/scf/test_uhf.py:73-90 (Coulson-Fischer point of stretched H2)
+    s = System("H 0 0 0\nH 0 0 2.7", "cc-pvtz", "cc-pvqz-jkfit", device=dev)
+    u = UHF(charge=0, ms=0.0, guess_mix=True)(s).run()
+    assert u.E == pytest.approx(-1.000027890395, abs=5e-8)
+    assert u.S2 == pytest.approx(0.9874559102128492, abs=5e-8)
+
+
 def test_uhf_water_triplet_golden(dev):
     s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit", device=dev)
     scf = UHF(charge=0, ms=1.0, e_tol=1e-10)(s).run()
