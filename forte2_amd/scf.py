@@ -12,7 +12,10 @@ Mirrors the reference's SCF driver and its defaults (forte2/scf/scf_base.py:
 
 Precision form (``precision="bf16x3"`` or ``"fp32"``): the J/K builds of the
 early iterations run in that precision on full densities; when |dE| drops
-below ``switch_inc`` an FP64 reference Fock matrix F_ref(D_ref) is built once
+below ``switch_inc`` (3e-3), or the energy rises by less than ten times that
+(the low-precision energy has reached its noise floor: at (H2O)_64 a BF16x3
+build is 2e-3 Eh off FP64 and consecutive |dE| stall near 3e-4 with
+alternating signs), an FP64 reference Fock matrix F_ref(D_ref) is built once
 and later iterations use incremental builds F = F_ref + G_lp(D - D_ref) whose
 error scales with |D - D_ref|; once |dE| < ``switch_fp64`` (1e-6) the
 remaining cycles, including the converged one, use FP64 builds.  The
@@ -142,7 +145,7 @@ def guess_mix(C, homo, angle=np.pi / 4):
 class _SCF:
     def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
                  d_tol=1e-6, maxiter=100, guess_type="minao", die_if_not_converged=True,
-                 precision="fp64", switch_inc=1e-4, switch_fp64=1e-6, incremental=True,
+                 precision="fp64", switch_inc=3e-3, switch_fp64=1e-6, incremental=True,
                  device=None, log=None, level_shift=None, level_shift_thresh=1e-5):
         self.charge = charge
         self.level_shift, self.level_shift_thresh = level_shift, level_shift_thresh
@@ -315,7 +318,8 @@ class _SCF:
             self.D = self._density()
             # stage transitions of the precision form (decided on the previous dE)
             next_stage = stage
-            if stage == "lp" and abs(self._last_dE) < self.switch_inc:
+            if stage == "lp" and (abs(self._last_dE) < self.switch_inc
+                                  or 0.0 < self._last_dE < 10.0 * self.switch_inc):
                 next_stage = "ref" if self.incremental else "fp64"
             elif stage == "inc" and abs(self._last_dE) < self.switch_fp64:
                 next_stage = "fp64"
