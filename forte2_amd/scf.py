@@ -338,7 +338,7 @@ class _SCF:
             self._last_dE = dE
             if abs(dE) < self.level_shift_thresh:  # scf_base.py:203-204
                 self._current_level_shift = None
-            dD = sum(float(torch.linalg.norm(d - do)) for d, do in zip(self.D, Dold))
+            dD = float(sum(torch.linalg.norm(d - do) for d, do in zip(self.D, Dold)))  # one host sync
             self.history.append((self.iter + 1, stage, self.E))
             if self.log:
                 self.log(f"{self.iter + 1:4d} {self.E:20.12f} {dE:12.4e} {dD:12.4e} {stage:>5s} {diis.status}")
