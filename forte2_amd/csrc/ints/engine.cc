@@ -880,10 +880,14 @@ int64_t pair_ncols(const f2ints_basis* bs, const std::pair<int, int>& ab) {
 // nsa (nsa + 1) / 2 components sa >= sb (row-major), so every (m, n) with
 // m >= n appears once.  The family pairs that hold these pairs are evaluated
 // whole; their other column pairs are not written.
-int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1, double* out,
-                     int64_t ldo) {
+// Rows P of the aux shells [s0, s1) (row 0 = the first function of shell s0),
+// packed columns of the pairs [q0, q1).
+int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, int s1, int64_t q0, int64_t q1,
+                     double* out, int64_t ldo) {
     const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
     if (q0 < 0 || q1 > (int64_t)pairs.size() || q0 > q1) return fail("f2ints_coulomb_3c_pairs: bad pair range");
+    if (s0 < 0 || s1 > (int)aux->shells.size() || s0 >= s1) return fail("f2ints_coulomb_3c_pairs: bad aux shell range");
+    const int64_t prow0 = aux->shells[s0].off;
     std::vector<int64_t> col(q1 - q0 + 1, 0);
     for (int64_t q = q0; q < q1; ++q) col[q - q0 + 1] = col[q - q0] + pair_ncols(bs, pairs[q]);
     if (ldo < col.back()) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the block's columns");
@@ -891,7 +895,7 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0
     for (int64_t q = q0; q < q1; ++q) fams.push_back(bs->fam_of_pair[q]);
     std::sort(fams.begin(), fams.end());
     fams.erase(std::unique(fams.begin(), fams.end()), fams.end());
-    const int maxlaux = max_aux_l(aux, 0, (int)aux->shells.size());
+    const int maxlaux = max_aux_l(aux, s0, s1);
 #pragma omp parallel num_threads(nthreads())
     {
         Scratch3c w(maxlaux, bs->maxl);
@@ -899,7 +903,8 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0
         for (size_t k = 0; k < fams.size(); ++k) {
             const FamPair& f = *static_cast<const FamPair*>(bs->fam_pairs[fams[k]]);
             const int na = nsph(bs->shells[f.a0].l), nb = nsph(bs->shells[f.b0].l), nab = na * nb;
-            for (const Shell& P : aux->shells) {
+            for (int sh = s0; sh < s1; ++sh) {
+                const Shell& P = aux->shells[sh];
                 const int nsP = nsph(P.l);
                 eval_fam(bs, f, P, w);
                 for (size_t c = 0; c < f.cols.size(); ++c) {
@@ -908,7 +913,7 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0
                     const bool diag = pairs[q].first == pairs[q].second;
                     double* o = out + col[q - q0];
                     for (int sP = 0; sP < nsP; ++sP) {
-                        double* row = o + (size_t)(P.off + sP) * ldo;
+                        double* row = o + (size_t)(P.off - prow0 + sP) * ldo;
                         const double* a = &w.acc[(c * nsP + sP) * nab];
                         if (!diag) {
                             std::copy(a, a + nab, row);
@@ -1098,8 +1103,15 @@ int f2ints_coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, doubl
 int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1,
                             double* out, int64_t ldo) {
     if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_pairs: null argument");
-    if (q0 == q1) return 0;
-    return coulomb_3c_pairs(aux, bs, q0, q1, out, ldo);
+    if (q0 == q1 || aux->shells.empty()) return 0;
+    return coulomb_3c_pairs(aux, bs, 0, (int)aux->shells.size(), q0, q1, out, ldo);
+}
+
+int f2ints_coulomb_3c_pairs_aux(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin, int shell_end,
+                                int64_t q0, int64_t q1, double* out, int64_t ldo) {
+    if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_pairs_aux: null argument");
+    if (q0 == q1 || shell_begin == shell_end) return 0;
+    return coulomb_3c_pairs(aux, bs, shell_begin, shell_end, q0, q1, out, ldo);
 }
 
 int f2ints_coulomb_4c(const f2ints_basis* b, double* out) {

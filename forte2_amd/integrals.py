@@ -100,6 +100,19 @@ def coulomb_3c_pairs(aux, basis, q0, q1, out):
     return out
 
 
+def coulomb_3c_pairs_aux(aux, basis, s0, s1, q0, q1, out):
+    """coulomb_3c_pairs for the aux shells [s0, s1) only: out (rows = the
+    functions of those shells, ldo) C-contiguous float64."""
+    offs = aux.shell_offsets()
+    if not (0 <= s0 < s1 <= len(offs) - 1):
+        raise ValueError("bad aux shell range")
+    if out.ndim != 2 or out.shape[0] != offs[s1] - offs[s0] or not out.flags.c_contiguous or out.dtype != np.float64:
+        raise ValueError("out has the wrong shape/layout")
+    _check(_lib().f2ints_coulomb_3c_pairs_aux(aux.handle, basis.handle, int(s0), int(s1), int(q0), int(q1),
+                                               _ptr(out), out.shape[1]))
+    return out
+
+
 def coulomb_3c_sum(aux, basis):
     """V[m, n] = sum_P (P|mn) over every function of `aux` (nbf x nbf)."""
     n = basis.size

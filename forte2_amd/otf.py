@@ -40,7 +40,7 @@ import numpy as np
 import torch
 
 from . import integrals as _ints
-from .df import _fitting_matrix, metric_factor
+from .df import _fitting_matrix, metric_factor, pair_columns
 from .jkbuilder import FockBuilder, _device_of
 
 
@@ -127,9 +127,20 @@ class FockBuilderOTF:
                                                    int(system.Zsum) // 2, max_shell)
         self.blocks = aux_shell_blocks(self.aux_sh_offsets, self.pblksize)
         N = self.nbf
-        self._dev_block = torch.empty((self.pblksize, N, N), dtype=torch.float64, device=self.device)
+        # The host evaluates each aux block over the significant shell pairs
+        # (m >= n) only, packed; the device scatters the columns to (m, n)
+        # and (n, m).  Every other element of a block is zero (the full-square
+        # routine screens the same pairs), so the block buffers are zeroed
+        # once and only the pair positions are ever written.
+        col0, mcol, ncol = pair_columns(self.basis)
+        self._npairs, self._ncols = len(col0) - 1, int(col0[-1])
+        self._idx_mn = torch.from_numpy(mcol * N + ncol).to(self.device)
+        self._idx_nm = torch.from_numpy(ncol * N + mcol).to(self.device)
+        self._dev_block = torch.zeros((self.pblksize, N, N), dtype=torch.float64, device=self.device)
+        w = max(1, self._ncols)
+        self._dev_packed = torch.empty((self.pblksize, w), dtype=torch.float64, device=self.device)
         pin = self.device.type == "cuda"
-        self._host = [torch.empty((self.pblksize, N, N), dtype=torch.float64, pin_memory=pin) for _ in range(2)]
+        self._host = [torch.empty((self.pblksize, w), dtype=torch.float64, pin_memory=pin) for _ in range(2)]
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         # a device FockBuilder whose B is re-pointed at each block
         self._kx = FockBuilder(B=self._dev_block, device=self.device)
@@ -166,7 +177,9 @@ class FockBuilderOTF:
     def _host_block(self, k, slot):
         s0, s1, p0, p1 = self.blocks[k]
         t0 = time.perf_counter()
-        _ints.coulomb_3c(self.auxbasis, self.basis, s0, s1, out=self._host[slot][: p1 - p0].numpy())
+        if self._ncols:
+            _ints.coulomb_3c_pairs_aux(self.auxbasis, self.basis, s0, s1, 0, self._npairs,
+                                       self._host[slot][: p1 - p0].numpy())
         self.host_seconds += time.perf_counter() - t0
 
     def _sweep(self):
@@ -197,14 +210,18 @@ class FockBuilderOTF:
                     fut = self._pool.submit(self._host_block, todo[it + 1], nxt)
                 nP = p1 - p0
                 if self.cache_bytes - sum(t.numel() * 8 for t in self._cache.values()) >= nP * N * N * 8:
-                    T = torch.empty((nP, N, N), dtype=torch.float64, device=self.device)
+                    T = torch.zeros((nP, N, N), dtype=torch.float64, device=self.device)
                     self._cache[k] = T
                 else:
                     T = self._dev_block[:nP]
-                T.copy_(self._host[slot][:nP], non_blocking=True)
+                packed = self._dev_packed[:nP, : self._ncols]
+                packed.copy_(self._host[slot][:nP, : self._ncols], non_blocking=True)
                 ev = torch.cuda.Event()
                 ev.record(stream)
                 copied[slot] = ev
+                Tv = T.view(nP, N * N)
+                Tv.index_copy_(1, self._idx_nm, packed)
+                Tv.index_copy_(1, self._idx_mn, packed)
                 it += 1
                 yield p0, p1, T
         finally:
@@ -364,5 +381,5 @@ class FockBuilderOTF:
         self._pool.shutdown(wait=True)
         self._kx.release()
         self._cache.clear()
-        self._dev_block = None
+        self._dev_block = self._dev_packed = None
         self._host = None

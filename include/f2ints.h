@@ -75,6 +75,13 @@ int64_t f2ints_ket_pairs(const f2ints_basis* bs, int* a, int* b, int64_t cap);
 int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t pair_begin,
                             int64_t pair_end, double* out, int64_t ldo);
 
+/* The same packed (P|ab) for the aux shells [shell_begin, shell_end) only:
+ * row 0 of out is the first function of shell_begin.  Used by the
+ * integral-direct builder for its aux-shell blocks of (P|mn)
+ * (forte2/jkbuilder/jkbuilder.py:676-740, 813-875). */
+int f2ints_coulomb_3c_pairs_aux(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin, int shell_end,
+                                int64_t pair_begin, int64_t pair_end, double* out, int64_t ldo);
+
 /* V[m][n] = sum over all functions P of `aux` of (P|mn), nbf x nbf: the
  * superposition-of-atomic-potentials guess potential of
  * forte2/scf/scf_utils.py:10-51 (aux = the SAP charge-density basis), without

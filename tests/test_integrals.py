@@ -172,6 +172,35 @@ def test_three_center_pair_blocks_match_square():
         ints.coulomb_3c_pairs(aux, b, 0, npairs, np.empty((aux.size, 4)))
 
 
+def test_three_center_pairs_by_aux_shells():
+    # f2ints_coulomb_3c_pairs_aux: the rows of the aux shells [s0, s1) of the
+    # packed unique half, bit for bit; the scatter to (m, n) and (n, m)
+    # rebuilds the square block (zero off the significant pairs in both)
+    from forte2_amd.df import pair_columns
+
+    atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5\nO 1 3.0 2 100.0 3 180.0")
+    aux, b = bs.build_basis("cc-pvtz-jkfit", atoms), bs.build_basis("cc-pvdz", atoms)
+    col0, m, n = pair_columns(b)
+    npairs, nc = len(col0) - 1, int(col0[-1])
+    whole = np.empty((aux.size, nc))
+    ints.coulomb_3c_pairs(aux, b, 0, npairs, whole)
+    offs = aux.shell_offsets()
+    nsh = len(offs) - 1
+    for s0, s1 in ((0, 1), (0, 7), (7, 20), (20, nsh)):
+        out = np.empty((offs[s1] - offs[s0], nc))
+        ints.coulomb_3c_pairs_aux(aux, b, s0, s1, 0, npairs, out)
+        assert np.array_equal(out, whole[offs[s0]:offs[s1]])
+        sq = ints.coulomb_3c(aux, b, s0, s1)
+        rebuilt = np.zeros_like(sq)
+        rebuilt[:, m, n] = out
+        rebuilt[:, n, m] = out
+        assert np.array_equal(rebuilt, sq)
+    with pytest.raises(ValueError):
+        ints.coulomb_3c_pairs_aux(aux, b, 3, 3, 0, npairs, np.empty((0, nc)))
+    with pytest.raises(ValueError):
+        ints.coulomb_3c_pairs_aux(aux, b, 0, 2, 0, npairs, np.empty((1, nc)))
+
+
 def test_three_center_sum_matches_blocks():
     # f2ints_coulomb_3c_sum = sum over P of the (P|mn) blocks (the SAP guess potential)
     atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5")
