@@ -147,6 +147,11 @@ class FockBuilderOTF:
         self.cache_bytes = int(cache_mb * 1024**2)
         self._cache = {}
         self.host_blocks_computed = 0  # blocks evaluated by the host engine so far
+        # With a single aux block, the later sweeps of one build (the J sweep
+        # after the K/gamma sweep) reuse the block left on the device instead
+        # of evaluating it again; each build still evaluates it once.
+        self._keep_resident = False
+        self._resident = False
         self.host_seconds = 0.0        # wall time of those evaluations (worker thread)
 
     # ------------------------------------------------------------------ metric
@@ -190,6 +195,11 @@ class FockBuilderOTF:
         pending host block is waited for and the copies out of the staging
         buffers are complete before the next sweep writes them again."""
         N = self.nbf
+        if self._resident and len(self.blocks) == 1 and 0 not in self._cache:
+            # the only block is still on the device from this build's first sweep
+            p0, p1 = self.blocks[0][2], self.blocks[0][3]
+            yield p0, p1, self._dev_block[: p1 - p0]
+            return
         todo = [k for k in range(len(self.blocks)) if k not in self._cache]
         copied = [None, None]  # event: the device copy out of that staging buffer is done
         fut = self._pool.submit(self._host_block, todo[0], 0) if todo else None
@@ -222,6 +232,7 @@ class FockBuilderOTF:
                 Tv = T.view(nP, N * N)
                 Tv.index_copy_(1, self._idx_nm, packed)
                 Tv.index_copy_(1, self._idx_mn, packed)
+                self._resident = self._keep_resident and T.data_ptr() == self._dev_block.data_ptr()
                 it += 1
                 yield p0, p1, T
         finally:
@@ -263,6 +274,13 @@ class FockBuilderOTF:
     def _run(self, Cs, Ds, want_k):
         """J of the densities Ds (None: no J) and K of the orbitals Cs (if
         want_k), sharing the sweeps over (P|mn)."""
+        self._keep_resident, self._resident = True, False
+        try:
+            return self._run_sweeps(Cs, Ds, want_k)
+        finally:
+            self._keep_resident = self._resident = False
+
+    def _run_sweeps(self, Cs, Ds, want_k):
         N, Q, dev = self.nbf, self.naux, self.device
         nd_j = len(Ds) if Ds is not None else 0
         J = [torch.zeros((N, N), dtype=torch.float64, device=dev) for _ in range(nd_j)]

@@ -122,6 +122,26 @@ def test_block_cache_gives_same_results(dev):
         assert np.array_equal(a[0], b[0])
 
 
+def test_single_block_is_evaluated_once_per_build(dev):
+    # with one aux block the J sweep reuses the block the K/gamma sweep left
+    # on the device; every build still evaluates it once
+    s = System(N2, "cc-pvdz", "cc-pvtz-jkfit", unit="bohr", device=dev)
+    fb = FockBuilderOTF(s, jk_mem_thres_mb=4000, device=dev)
+    assert len(fb.blocks) == 1
+    C = _orth(s.nbf, 7, 8)
+    J1, K1 = fb.build_JK([C])
+    assert fb.host_blocks_computed == 1
+    J2, K2 = fb.build_JK([C])
+    assert fb.host_blocks_computed == 2
+    assert np.array_equal(J1[0], J2[0]) and np.array_equal(K1[0], K2[0])
+    fi = FockBuilder(s, device=s.device)
+    Ji, Ki = fi.build_JK([C])
+    assert _err(J1[0], Ji[0]) <= TOL and _err(K1[0], Ki[0]) <= TOL
+    Jd = fb.build_J([C @ C.T])[0]  # J alone: two sweeps, one evaluation
+    assert fb.host_blocks_computed == 3 and _err(Jd, Ji[0]) <= TOL
+    fi.release()
+
+
 def test_eigen_metric_branch(dev):
     s = System(N2, "cc-pvdz", "cc-pvtz-jkfit", unit="bohr", df_ortho_rtol=1e-10, device=dev)
     fb = FockBuilderOTF(s, jk_mem_thres_mb=0.5, device=dev)
