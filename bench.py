@@ -360,15 +360,22 @@ def cpu_baseline(B, C, q_sample):
     cores = host_cores()
     Bs = B[:q_sample].cpu().numpy()
     Ch = C.cpu().numpy()
-    with threadpool_limits(limits=cores):
-        BnPm = np.ascontiguousarray(Bs.transpose((2, 0, 1)))
-        ojk.build_JK(Bs, [Ch], BnPm)  # warm-up
+    D = Ch @ Ch.T
+
+    def median_time(fn):
+        fn()  # warm-up
         ts = []
         for _ in range(3):
             t0 = time.perf_counter()
-            ojk.build_JK(Bs, [Ch], BnPm)
+            fn()
             ts.append(time.perf_counter() - t0)
-    t = float(np.median(ts))
+        return float(np.median(ts))
+
+    with threadpool_limits(limits=cores):
+        BnPm = np.ascontiguousarray(Bs.transpose((2, 0, 1)))
+        t = median_time(lambda: ojk.build_JK(Bs, [Ch], BnPm))
+        tj = median_time(lambda: ojk.build_J(Bs, [D]))
+        tk = median_time(lambda: ojk.build_K_nPm(Bs, [Ch], BnPm))
     Q = B.shape[0]
     return {
         "value": (1.0 / t) * q_sample / Q,
@@ -378,7 +385,22 @@ def cpu_baseline(B, C, q_sample):
         "sample": f"oracle build_JK (numpy FP64, B_nPm path) on B[:{q_sample}] of the same "
                   f"(H2O)_n tensor (naux {Q}), median of 3 after 1 warm-up, "
                   f"{t:.2f} s/sample-build, scaled by {q_sample}/{Q}",
+        "j_only_builds_per_s": (1.0 / tj) * q_sample / Q,
+        "k_only_builds_per_s": (1.0 / tk) * q_sample / Q,
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
