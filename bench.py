@@ -448,6 +448,8 @@ def main():
     print(f"[bench] fp64: {t64*1e3:.2f} ms/build", file=sys.stderr, flush=True)
     err_j = float((Jm - J64).abs().max())
     err_k = float((Km - K64).abs().max())
+    rel_j = err_j / max(float(J64.abs().max()), 1e-300)
+    rel_k = err_k / max(float(K64.abs().max()), 1e-300)
 
     ph_mix, la_mix = kernel_times(fb, C, args.mixed)
     ph64, la64 = kernel_times(fb, C, "fp64")
@@ -497,6 +499,9 @@ def main():
         "fp64": {"builds_per_s": 1.0 / t64, "ms_per_build": t64 * 1e3},
         "speedup_vs_fp64": t64 / t_mix,
         "max_abs_diff_vs_fp64": {"J": err_j, "K": err_k},
+        # relative to the largest FP64 element; the SCF-level error of the
+        # precision form is measured on real integrals by --scf
+        "max_rel_diff_vs_fp64": {"J": rel_j, "K": rel_k},
         "phases_ms": {args.mixed: ph_mix, "fp64": ph64},
         "J": {
             "GBps": jb_mix / (j_ms * 1e-3) / 1e9, "fp64_GBps": jb64 / (j64_ms * 1e-3) / 1e9,
