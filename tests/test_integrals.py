@@ -6,7 +6,9 @@ Golden values: tests/integrals/test_two_electron.py:10-46 (25 STO-3G water
 (P|Q), (m|P), (P|mn)), all at the reference's tolerances (abs 5e-8, rel 1e-6).
 """
 
+import json
 import math
+import os
 
 import mpmath
 import numpy as np
@@ -16,25 +18,14 @@ from forte2_amd import basis as bs
 from forte2_amd import geometry as geo
 from forte2_amd import integrals as ints
 
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+NORMS = GOLD["norms"]
+
 WATER_STO3G = """O 0.000000000000     0.000000000000    -0.061664597388
 H 0.000000000000    -0.711620616370     0.489330954643
 H 0.000000000000     0.711620616370     0.489330954643"""
 
-STO3G_ERI = {
-    (4, 5, 0, 4): 0.011203183573992602, (6, 3, 0, 1): 0.043937066018280665,
-    (6, 1, 0, 1): 0.10316476705668452, (1, 3, 1, 5): 0.0347130867494285,
-    (3, 2, 2, 5): 0.012205381062317456, (0, 4, 4, 5): 0.011203183573992604,
-    (0, 4, 5, 4): 0.011203183573992604, (2, 5, 2, 1): 0.088959722367807,
-    (1, 1, 6, 0): 0.0669877231256632, (2, 3, 3, 5): -0.013249429680865767,
-    (2, 3, 6, 3): 0.013249429680865767, (3, 5, 1, 6): 0.08488308941086266,
-    (5, 0, 0, 5): 0.009602115579028458, (1, 2, 3, 5): -0.01955131963900513,
-    (0, 2, 5, 2): 0.014154072956092055, (5, 5, 5, 1): 0.33820230329917145,
-    (1, 3, 0, 5): 0.0018694062463856086, (0, 0, 0, 0): 4.785065751815717,
-    (0, 5, 3, 0): 0.0010038501294464502, (3, 6, 1, 1): 0.163693540728617,
-    (3, 3, 2, 2): 0.785270203138278, (3, 2, 3, 6): 0.013249429680865767,
-    (0, 3, 1, 5): 0.004687753192919011, (1, 0, 6, 2): 0.056745507084677765,
-    (0, 5, 5, 3): 0.012314281459575225,
-}
+STO3G_ERI = {tuple(k): v for k, v in GOLD["sto3g_water_eri"]["values"]}  # tests/integrals/test_two_electron.py:10-46
 
 
 def test_sto3g_water_eri_golden():
@@ -50,7 +41,7 @@ def test_sto3g_water_eri_golden():
 def test_three_center_norm_golden():
     atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5")
     P = ints.coulomb_3c(bs.build_basis("cc-pvtz-jkfit", atoms), bs.build_basis("cc-pvdz", atoms))
-    assert np.linalg.norm(P) == pytest.approx(60.4085268377979, rel=1e-6)
+    assert np.linalg.norm(P) == pytest.approx(NORMS["water_dz_tzjkfit_3c"]["value"], rel=1e-6)
     assert np.abs(P - P.transpose(0, 2, 1)).max() == 0.0
 
 
@@ -58,8 +49,8 @@ def test_two_center_norms_golden():
     atoms = geo.parse_geometry("N 0 0 0\nN 0 0 1.1")
     b, a = bs.build_basis("cc-pvdz", atoms), bs.build_basis("cc-pvtz-jkfit", atoms)
     M = ints.coulomb_2c(a)
-    assert np.linalg.norm(M) == pytest.approx(159.31789654133004, rel=1e-6)
-    assert np.linalg.norm(ints.coulomb_2c(b, a)) == pytest.approx(149.6527772441834, rel=1e-6)
+    assert np.linalg.norm(M) == pytest.approx(NORMS["n2_2c_metric"]["value"], rel=1e-6)
+    assert np.linalg.norm(ints.coulomb_2c(b, a)) == pytest.approx(NORMS["n2_2c_mixed"]["value"], rel=1e-6)
     assert np.linalg.eigvalsh(M).min() > 0  # Coulomb metric is positive definite
 
 

@@ -6,6 +6,9 @@ numpy in / numpy out, empty orbital sets, and the reference's DF-MP2 golden
 energy of water in cc-pVQZ / cc-pVQZ-JKFIT (tests/mp2/test_mp2.py:8-54).
 """
 
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -13,6 +16,9 @@ import torch
 from forte2_amd.jkbuilder import FockBuilder
 from forte2_amd.workloads import orthonormal_C, synthetic_B
 from oracle import jk as ojk
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+EREF = GOLD["energies"]
 
 pytestmark = pytest.mark.gpu
 
@@ -97,7 +103,7 @@ def test_mp2_golden(dev):
     print("[mo-gpu] water cc-pVQZ MP2", flush=True)
     s = System(water, "cc-pvqz", "cc-pvqz-jkfit", device=dev)
     scf = RHF(charge=0, e_tol=1e-10)(s).run()
-    assert scf.E == pytest.approx(-76.0614664072629836, abs=5e-8)
+    assert scf.E == pytest.approx(EREF["water_qz_mp2"]["E_scf"], abs=5e-8)
     fb = s.fock_builder
     no = scf.na
     C = scf.C[0]
@@ -114,7 +120,7 @@ def test_mp2_golden(dev):
     den = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
     emp2 = scf.E + np.sum(V * (2.0 * V - V.transpose(0, 1, 3, 2)) / den)
     print(f"   E(MP2) = {emp2:.12f}", flush=True)
-    assert emp2 == pytest.approx(-76.3710978841482984, abs=5e-8)
+    assert emp2 == pytest.approx(EREF["water_qz_mp2"]["E_total"], abs=5e-8)
 
 
 def test_ao_jk_equal_mo_reconstruction(dev):

@@ -8,6 +8,9 @@ the truncated-eigen metric, the (P|mn) block cache, and an RHF run through
 the System factory that ends on the reference's golden energy.
 """
 
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -16,6 +19,9 @@ from forte2_amd.jkbuilder import FockBuilder
 from forte2_amd.otf import FockBuilderOTF
 from forte2_amd.system import System
 from oracle import otf as ootf
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+EREF = GOLD["energies"]
 
 pytestmark = pytest.mark.gpu
 
@@ -165,7 +171,7 @@ def test_rhf_through_factory(dev):
     assert isinstance(s.fock_builder, FockBuilderOTF)
     scf = RHF(charge=0, e_tol=1e-10)(s).run()
     print(f"   E = {scf.E:.12f}", flush=True)
-    assert scf.E == pytest.approx(-76.02176598836786, abs=5e-8)
+    assert scf.E == pytest.approx(EREF["water_dz_rhf"]["E"], abs=5e-8)
 
 
 def test_cl2_golden_through_otf(dev):
@@ -179,7 +185,7 @@ def test_cl2_golden_through_otf(dev):
     assert len(s.fock_builder.blocks) > 1
     scf = RHF(charge=0)(s).run()
     print(f"   E = {scf.E:.12f}", flush=True)
-    assert scf.E == pytest.approx(-918.943349338796, abs=5e-8)
+    assert scf.E == pytest.approx(EREF["cl2_qz_rhf"]["E"], abs=5e-8)
 
 
 def test_errors(dev):

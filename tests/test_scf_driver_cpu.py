@@ -2,6 +2,9 @@
 builder: checks the loop, the precision schedule and the build-request
 protocol without a GPU.  The builder's "low precision" rounds B to FP32."""
 
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -10,6 +13,9 @@ from forte2_amd.scf import RHF, UHF
 from forte2_amd.system import System
 from oracle import jk as ojk
 from oracle import scf as oscf
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+EREF = GOLD["energies"]
 
 WATER = """O 0.000000000000 0.000000000000 -0.061664597388
 H 0.000000000000 -0.711620616369 0.489330954643
@@ -91,8 +97,8 @@ def test_guess_mix_matches_oracle():
     ref = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum, 0.0,
                        X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10, guess_mix=True)
     assert abs(u.E - ref.E) < 1e-10 and u.iter == ref.iter
-    assert u.E == pytest.approx(-1.000027890395, abs=5e-8)
-    assert u.S2 == pytest.approx(0.9874559102128492, abs=5e-8)
+    assert u.E == pytest.approx(EREF["h2_stretched_tz_uhf_guess_mix"]["E"], abs=5e-8)
+    assert u.S2 == pytest.approx(EREF["h2_stretched_tz_uhf_guess_mix"]["S2"], abs=5e-8)
 
 
 def test_level_shift_validation(water):

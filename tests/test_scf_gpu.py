@@ -5,6 +5,9 @@ same integrals and guess (water; (H2O)_16; an alkane cation).  Precision form:
 the final energy equals the device FP64 energy to <= 1e-8 Eh.
 """
 
+import json
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -15,14 +18,17 @@ from forte2_amd.system import System
 from oracle import jk as ojk
 from oracle import scf as oscf
 
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+EREF = GOLD["energies"]
+
 pytestmark = pytest.mark.gpu
 
 WATER = """O            0.000000000000     0.000000000000    -0.061664597388
 H            0.000000000000    -0.711620616369     0.489330954643
 H            0.000000000000     0.711620616369     0.489330954643"""
-E_WATER_DZ = -76.02176598836786       # tests/orbitals/test_cube.py:15
-E_WATER_QZ_UHF = -75.810772399321     # tests/scf/test_uhf.py:11
-S2_WATER_QZ_UHF = 2.005739321130
+E_WATER_DZ = EREF["water_dz_rhf"]["E"]       # tests/orbitals/test_cube.py:15
+E_WATER_QZ_UHF = EREF["water_qz_uhf_triplet"]["E"]     # tests/scf/test_uhf.py:11
+S2_WATER_QZ_UHF = EREF["water_qz_uhf_triplet"]["S2"]
 
 
 def _oracle(s, ms=None, charge=0, e_tol=1e-10):
@@ -86,7 +92,7 @@ def test_rhf_level_shift_golden(dev):
     s = System(xyz, "cc-pvtz", "cc-pvtz-jkfit", device=dev)
     scf = RHF(charge=0, level_shift=0.5, do_diis=False)(s).run()
     print(f"   E = {scf.E:.12f} iters {scf.iter}", flush=True)
-    assert scf.E == pytest.approx(-169.001135897278, abs=5e-8)
+    assert scf.E == pytest.approx(EREF["formamide_tz_rhf_level_shift"]["E"], abs=5e-8)
     assert scf.iter < 40
 
 
@@ -94,8 +100,8 @@ def test_uhf_guess_mix_golden(dev):
     # tests/scf/test_uhf.py:73-90 (Coulson-Fischer point of stretched H2)
     s = System("H 0 0 0\nH 0 0 2.7", "cc-pvtz", "cc-pvqz-jkfit", device=dev)
     u = UHF(charge=0, ms=0.0, guess_mix=True)(s).run()
-    assert u.E == pytest.approx(-1.000027890395, abs=5e-8)
-    assert u.S2 == pytest.approx(0.9874559102128492, abs=5e-8)
+    assert u.E == pytest.approx(EREF["h2_stretched_tz_uhf_guess_mix"]["E"], abs=5e-8)
+    assert u.S2 == pytest.approx(EREF["h2_stretched_tz_uhf_guess_mix"]["S2"], abs=5e-8)
 
 
 def test_uhf_water_triplet_golden(dev):
@@ -170,5 +176,5 @@ def test_uhf_one_electron_golden(dev):
     s = System("H 0 0 0\nH 0 0 1.0", "cc-pvqz", "cc-pvqz-jkfit", device=dev)
     scf = UHF(charge=1, ms=-0.5, e_tol=1e-10)(s).run()
     print(f"   E = {scf.E:.12f} <S^2> = {scf.S2:.6f}", flush=True)
-    assert scf.E == pytest.approx(-0.601864064744, abs=5e-8)
+    assert scf.E == pytest.approx(EREF["h2plus_qz_uhf"]["E"], abs=5e-8)
     assert scf.S2 == pytest.approx(0.75, abs=5e-8)
