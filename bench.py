@@ -365,7 +365,7 @@ def cpu_baseline(B, C, q_sample):
     def median_time(fn):
         fn()  # warm-up
         ts = []
-        for _ in range(3):
+        for _ in range(5):
             t0 = time.perf_counter()
             fn()
             ts.append(time.perf_counter() - t0)
@@ -383,7 +383,7 @@ def cpu_baseline(B, C, q_sample):
         "cores": cores,
         "kind": "port",
         "sample": f"oracle build_JK (numpy FP64, B_nPm path) on B[:{q_sample}] of the same "
-                  f"(H2O)_n tensor (naux {Q}), median of 3 after 1 warm-up, "
+                  f"(H2O)_n tensor (naux {Q}), median of 5 after 1 warm-up, "
                   f"{t:.2f} s/sample-build, scaled by {q_sample}/{Q}",
         "j_only_builds_per_s": (1.0 / tj) * q_sample / Q,
         "k_only_builds_per_s": (1.0 / tk) * q_sample / Q,
