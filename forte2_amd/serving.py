@@ -245,7 +245,7 @@ class SCFServer:
         difference density, and jobs the grouped path does not take, run on
         their own.
         """
-        done, results, solo = [], {}, []
+        done, results, solo, failed = [], {}, [], set()
         groups = {}
         for jb in active:
             kind, args, prec = jb.pending
@@ -259,7 +259,7 @@ class SCFServer:
                 grp = jobs[g0:g0 + MAX_GROUP]
                 try:
                     outs = self._build_group(grp)
-                except ValueError:  # the library declines the group: these jobs run alone from now on
+                except Exception:  # the library declines or fails the group: these jobs run alone from now on
                     for jb in grp:
                         jb._groupable[(kind, prec, nd)] = False
                     solo.extend(grp)
@@ -268,14 +268,25 @@ class SCFServer:
                     results[id(jb)] = out
                 self.group_sizes.append(len(grp))
         for jb in solo:
-            results[id(jb)] = jb.scf._execute(jb.pending)
+            try:
+                results[id(jb)] = jb.scf._execute(jb.pending)
+            except Exception as e:  # this job's build failed: it ends, the others go on
+                jb.error = repr(e)
+                failed.add(id(jb))
+                continue
             self.group_sizes.append(1)
         for jb in active:
+            if id(jb) in failed:
+                done.append(jb)
+                continue
             try:
                 if jb.send(results[id(jb)]):
                     done.append(jb)
             except RuntimeError as e:  # non-convergence: this job fails, the others go on
                 jb.error = str(e)
+                done.append(jb)
+            except Exception as e:  # any other failure of this job's step: likewise
+                jb.error = repr(e)
                 done.append(jb)
         self.ticks += 1
         return done
@@ -367,7 +378,7 @@ class SCFServer:
                     for jb in self._tick(active):
                         active.remove(jb)
                         res = self.results[jb.req.id]
-                        res.E, res.iterations = jb.scf.E, jb.scf.iter
+                        res.E, res.iterations = getattr(jb.scf, "E", None), getattr(jb.scf, "iter", 0)
                         res.converged, res.error = bool(jb.scf.converged), jb.error
                         res.finished = now()
                         used -= jb.footprint

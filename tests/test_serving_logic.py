@@ -27,3 +27,39 @@ def test_footprint_is_dominated_by_B_and_monotone():
 def test_request_defaults():
     r = SCFRequest(0, "H 0 0 0\nH 0 0 0.74")
     assert r.basis == "cc-pvdz" and r.aux == "cc-pvtz-jkfit" and r.ms is None
+
+
+def test_a_failing_step_ends_only_its_job():
+    # a build or an SCF step that raises ends that job in the tick; the other
+    # jobs of the tick get their results and go on
+    import torch
+
+    from forte2_amd.serving import SCFServer
+
+    class _SCF:
+        def __init__(self, fail):
+            self.fail = fail
+
+        def _execute(self, pending):
+            if self.fail == "build":
+                raise MemoryError("workspace")
+            return [torch.zeros(2, 2)]
+
+    class _Job:
+        def __init__(self, fail=None):
+            self.scf, self.pending, self.error, self.got = _SCF(fail), ("J", [torch.zeros(2, 2)], "fp64"), None, None
+            self.fail = fail
+
+        def send(self, result):
+            if self.fail == "step":
+                raise ValueError("bad step")
+            self.got = result
+            return False
+
+    srv = SCFServer(device="cpu", hbm_budget=1 << 30, grouped=False)
+    jobs = [_Job(), _Job("build"), _Job("step"), _Job()]
+    done = srv._tick(jobs)
+    assert done == [jobs[1], jobs[2]]
+    assert "MemoryError" in jobs[1].error and "bad step" in jobs[2].error
+    assert jobs[0].got is not None and jobs[3].got is not None and jobs[0].error is None
+
