@@ -251,7 +251,7 @@ const std::vector<RStep>& r_steps() {
 void compute_R(int L, double alpha, double X, double Y, double Z, double* R0, double* work) {
     const int nh = nherm(L);
     double F[NBOYS + 2];
-    boys().eval(L, alpha * (X * X + Y * Y + Z * Z), F);
+    boys().eval(L, alpha * std::fma(Z, Z, std::fma(Y, Y, X * X)), F);
     // work[n * nh + idx]
     double m2a = 1.0;
     for (int n = 0; n <= L; ++n) {
@@ -265,12 +265,49 @@ void compute_R(int L, double alpha, double X, double Y, double Z, double* R0, do
         const double Pd = P[st.dir];
         for (int n = 0; n <= L - st.N; ++n) {
             const double* Rn1 = work + (size_t)(n + 1) * nh;
-            double val = Pd * Rn1[st.a];
-            if (st.b >= 0) val += st.k1 * Rn1[st.b];
-            work[(size_t)n * nh + i] = val;
+            const double val = Pd * Rn1[st.a];
+            work[(size_t)n * nh + i] = st.b >= 0 ? std::fma(st.k1, Rn1[st.b], val) : val;
         }
     }
     std::memcpy(R0, work, sizeof(double) * nh);
+}
+
+// compute_R for kLanes independent (alpha, X, Y, Z): R4[idx * kLanes + lane],
+// work4 of (L + 1) * nherm(L) * kLanes.  Every lane does the arithmetic of
+// compute_R in the same order, so each lane's values are those of a call of
+// compute_R; the lane loops vectorise.
+constexpr int kLanes = 4;
+
+void compute_R4(int L, const double* alpha, const double* X, const double* Y, const double* Z, double* R4,
+                double* work4) {
+    const int nh = nherm(L);
+    double F[kLanes][NBOYS + 2];
+    for (int l = 0; l < kLanes; ++l) boys().eval(L, alpha[l] * std::fma(Z[l], Z[l], std::fma(Y[l], Y[l], X[l] * X[l])), F[l]);
+    double m2a[kLanes];
+    for (int l = 0; l < kLanes; ++l) m2a[l] = 1.0;
+    for (int n = 0; n <= L; ++n)
+        for (int l = 0; l < kLanes; ++l) {
+            work4[((size_t)n * nh + 0) * kLanes + l] = m2a[l] * F[l][n];
+            m2a[l] *= -2.0 * alpha[l];
+        }
+    const RStep* steps = r_steps().data();
+    const double* P[3] = {X, Y, Z};
+    for (int i = 1; i < nh; ++i) {
+        const RStep& st = steps[i];
+        const double* Pd = P[st.dir];
+        for (int n = 0; n <= L - st.N; ++n) {
+            const double* Rn1 = work4 + (size_t)(n + 1) * nh * kLanes;
+            double* out = work4 + ((size_t)n * nh + i) * kLanes;
+            const double* ra = Rn1 + (size_t)st.a * kLanes;
+            if (st.b >= 0) {
+                const double* rb = Rn1 + (size_t)st.b * kLanes;
+                for (int l = 0; l < kLanes; ++l) out[l] = std::fma(st.k1, rb[l], Pd[l] * ra[l]);
+            } else {
+                for (int l = 0; l < kLanes; ++l) out[l] = Pd[l] * ra[l];
+            }
+        }
+    }
+    std::memcpy(R4, work4, sizeof(double) * nh * kLanes);
 }
 
 // ---------------------------------------------------------------- basis
@@ -693,12 +730,17 @@ const std::vector<std::pair<int, int>>& ket_pairs(const f2ints_basis* cbs) {
 // scratch of one thread for eval_fam
 struct Scratch3c {
     std::vector<double> R, work, Y, V, acc;
+    std::vector<double> R4, work4, Y4, V4, acc4[kLanes];  // lane-batched (eval_fam4)
     Scratch3c(int maxlaux, int maxl) {
         const int Lmax = maxlaux + 2 * maxl;
         R.resize(nherm(Lmax));
         work.resize((size_t)(Lmax + 1) * nherm(Lmax));
         Y.resize((size_t)nsph(maxlaux) * nherm(2 * maxl));
         V.resize((size_t)nsph(maxlaux) * nsph(maxl) * nsph(maxl));
+        R4.resize(R.size() * kLanes);
+        work4.resize(work.size() * kLanes);
+        Y4.resize(Y.size() * kLanes);
+        V4.resize(V.size() * kLanes);
     }
 };
 
@@ -756,7 +798,7 @@ void eval_fam(const f2ints_basis* bs, const FamPair& f, const Shell& P, Scratch3
                 std::fill(y, y + nhab, 0.0);
                 for (const SphTerm& tp : Hm.sparse[lP][sP]) {
                     const int* ix = &yix[(size_t)tp.cart * nhab];
-                    for (int h = 0; h < nhab; ++h) y[h] += tp.c * w.R[ix[h]];
+                    for (int h = 0; h < nhab; ++h) y[h] = std::fma(tp.c, w.R[ix[h]], y[h]);
                 }
             }
             // V[sP][s2] = sum_h E[s2][h] Y[sP][h], shared by the column pairs;
@@ -769,17 +811,103 @@ void eval_fam(const f2ints_basis* bs, const FamPair& f, const Shell& P, Scratch3
                 for (int h = 0; h < nhab; ++h) {
                     const double yh = y[h];
                     const double* e = &pp.E[(size_t)h * nab];
-                    for (int s2 = 0; s2 < nab; ++s2) v[s2] += e[s2] * yh;
+                    for (int s2 = 0; s2 < nab; ++s2) v[s2] = std::fma(e[s2], yh, v[s2]);
                 }
             }
             for (int c = 0; c < ncp; ++c) {
                 if (pp.w[c] == 0.0) continue;
                 const double pref = base * pp.w[c];
                 double* a = &w.acc[(size_t)c * nsP * nab];
-                for (int i = 0; i < nsP * nab; ++i) a[i] += pref * w.V[i];
+                for (int i = 0; i < nsP * nab; ++i) a[i] = std::fma(pref, w.V[i], a[i]);
             }
         }
     }
+}
+
+// eval_fam for kLanes aux shells of one angular momentum with one primitive
+// each: w.acc4[lane] as w.acc of eval_fam(.., *Ps[lane], w), lane by lane the
+// same arithmetic in the same order (bit for bit the same values); R, the
+// aux-side transform and the E contraction run across the lanes.
+void eval_fam4(const f2ints_basis* bs, const FamPair& f, const Shell* const* Ps, Scratch3c& w) {
+    const Harmonics& Hm = harm();
+    const Shell& A = bs->shells[f.a0];
+    const Shell& B = bs->shells[f.b0];
+    const int lab = A.l + B.l, nhab = nherm(lab), nab = nsph(A.l) * nsph(B.l);
+    const int lP = Ps[0]->l, nsP = nsph(lP), L = lP + lab;
+    const int ncp = (int)f.cols.size();
+    for (int l = 0; l < kLanes; ++l) w.acc4[l].assign((size_t)ncp * nsP * nab, 0.0);
+    static const double two_pi52 = 2.0 * std::pow(PI, 2.5);
+    const std::vector<int>& yix = y_index(lP, lab);
+    double g[kLanes], gl[kLanes], coef[kLanes];
+    for (int l = 0; l < kLanes; ++l) {
+        g[l] = Ps[l]->alpha[0];
+        gl[l] = std::pow(2.0 * g[l], -lP);
+        coef[l] = Ps[l]->coef[0];
+    }
+    double al[kLanes], X[kLanes], Y[kLanes], Z[kLanes], base[kLanes];
+    for (const FamPrim& pp : f.prims) {
+        const double p = pp.p;
+        for (int l = 0; l < kLanes; ++l) {
+            al[l] = g[l] * p / (g[l] + p);
+            X[l] = Ps[l]->A[0] - pp.P[0];
+            Y[l] = Ps[l]->A[1] - pp.P[1];
+            Z[l] = Ps[l]->A[2] - pp.P[2];
+            base[l] = two_pi52 / (g[l] * p * std::sqrt(g[l] + p)) * coef[l] * gl[l];
+        }
+        compute_R4(L, al, X, Y, Z, w.R4.data(), w.work4.data());
+        // Y4[(sP * nhab + h) * kLanes + lane]
+        for (int sP = 0; sP < nsP; ++sP) {
+            double* y = &w.Y4[(size_t)sP * nhab * kLanes];
+            std::fill(y, y + (size_t)nhab * kLanes, 0.0);
+            for (const SphTerm& tp : Hm.sparse[lP][sP]) {
+                const int* ix = &yix[(size_t)tp.cart * nhab];
+                for (int h = 0; h < nhab; ++h) {
+                    const double* r = &w.R4[(size_t)ix[h] * kLanes];
+                    for (int l = 0; l < kLanes; ++l) y[h * kLanes + l] = std::fma(tp.c, r[l], y[h * kLanes + l]);
+                }
+            }
+        }
+        // V4[(sP * nab + s2) * kLanes + lane], h ascending as in eval_fam
+        for (int sP = 0; sP < nsP; ++sP) {
+            const double* y = &w.Y4[(size_t)sP * nhab * kLanes];
+            double* v = &w.V4[(size_t)sP * nab * kLanes];
+            std::fill(v, v + (size_t)nab * kLanes, 0.0);
+            for (int h = 0; h < nhab; ++h) {
+                const double* e = &pp.E[(size_t)h * nab];
+                const double* yh = y + (size_t)h * kLanes;
+                for (int s2 = 0; s2 < nab; ++s2)
+                    for (int l = 0; l < kLanes; ++l) v[s2 * kLanes + l] = std::fma(e[s2], yh[l], v[s2 * kLanes + l]);
+            }
+        }
+        for (int c = 0; c < ncp; ++c) {
+            if (pp.w[c] == 0.0) continue;
+            for (int l = 0; l < kLanes; ++l) {
+                const double pref = base[l] * pp.w[c];
+                double* a = &w.acc4[l][(size_t)c * nsP * nab];
+                for (int i = 0; i < nsP * nab; ++i) a[i] = std::fma(pref, w.V4[(size_t)i * kLanes + l], a[i]);
+            }
+        }
+    }
+}
+
+// The aux shells [s0, s1) in batches: runs of kLanes shells of one angular
+// momentum with one primitive each (taken in order of l, then of index), and
+// the rest one by one.  Each batch is a list of shell indices.
+std::vector<std::vector<int>> aux_batches(const f2ints_basis* aux, int s0, int s1) {
+    std::vector<std::vector<int>> out;
+    std::vector<std::vector<int>> by_l(LMAX + 1);
+    for (int s = s0; s < s1; ++s) {
+        const Shell& P = aux->shells[s];
+        if (P.alpha.size() == 1) by_l[P.l].push_back(s);
+        else out.push_back({s});
+    }
+    for (int l = 0; l <= LMAX; ++l) {
+        const std::vector<int>& v = by_l[l];
+        size_t i = 0;
+        for (; i + kLanes <= v.size(); i += kLanes) out.emplace_back(v.begin() + i, v.begin() + i + kLanes);
+        for (; i < v.size(); ++i) out.push_back({v[i]});
+    }
+    return out;
 }
 
 int max_aux_l(const f2ints_basis* aux, int sh0, int sh1) {
@@ -896,6 +1024,7 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, in
     std::sort(fams.begin(), fams.end());
     fams.erase(std::unique(fams.begin(), fams.end()), fams.end());
     const int maxlaux = max_aux_l(aux, s0, s1);
+    const std::vector<std::vector<int>> batches = aux_batches(aux, s0, s1);
 #pragma omp parallel num_threads(nthreads())
     {
         Scratch3c w(maxlaux, bs->maxl);
@@ -903,10 +1032,8 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, in
         for (size_t k = 0; k < fams.size(); ++k) {
             const FamPair& f = *static_cast<const FamPair*>(bs->fam_pairs[fams[k]]);
             const int na = nsph(bs->shells[f.a0].l), nb = nsph(bs->shells[f.b0].l), nab = na * nb;
-            for (int sh = s0; sh < s1; ++sh) {
-                const Shell& P = aux->shells[sh];
+            auto store = [&](const Shell& P, const std::vector<double>& acc) {
                 const int nsP = nsph(P.l);
-                eval_fam(bs, f, P, w);
                 for (size_t c = 0; c < f.cols.size(); ++c) {
                     const int64_t q = f.q[c];
                     if (q < q0 || q >= q1) continue;
@@ -914,7 +1041,7 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, in
                     double* o = out + col[q - q0];
                     for (int sP = 0; sP < nsP; ++sP) {
                         double* row = o + (size_t)(P.off - prow0 + sP) * ldo;
-                        const double* a = &w.acc[(c * nsP + sP) * nab];
+                        const double* a = &acc[(c * nsP + sP) * nab];
                         if (!diag) {
                             std::copy(a, a + nab, row);
                         } else {
@@ -923,6 +1050,18 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, in
                                 for (int sb = 0; sb <= sa; ++sb) row[t++] = a[sa * nb + sb];
                         }
                     }
+                }
+            };
+            for (const std::vector<int>& bt : batches) {
+                if (bt.size() == (size_t)kLanes) {
+                    const Shell* Ps[kLanes];
+                    for (int l = 0; l < kLanes; ++l) Ps[l] = &aux->shells[bt[l]];
+                    eval_fam4(bs, f, Ps, w);
+                    for (int l = 0; l < kLanes; ++l) store(*Ps[l], w.acc4[l]);
+                } else {
+                    const Shell& P = aux->shells[bt[0]];
+                    eval_fam(bs, f, P, w);
+                    store(P, w.acc);
                 }
             }
         }

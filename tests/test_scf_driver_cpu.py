@@ -64,8 +64,11 @@ def test_plain_form_matches_oracle(water):
     assert abs(r.E - ref.E) < 1e-10
     assert r.iter == ref.iter
     # the final iteration diagonalises the new Fock matrix, which moves the
-    # density once more (scf_base.py:214-221)
-    assert r.E != r.history[-1][2] and abs(r.E - r.history[-1][2]) < 1e-10
+    # density once more (scf_base.py:214-221); at a loose threshold the move
+    # shows in the energy (a repeat of the last diagonalisation would leave
+    # it bit for bit)
+    loose = RHF(charge=0, e_tol=1e-5, d_tol=1e-2, do_diis=False)(s).run()
+    assert 1e-12 < abs(loose.E - loose.history[-1][2]) < 1e-4
 
 
 @pytest.mark.parametrize("method", ["rhf", "uhf"])
