@@ -962,6 +962,7 @@ int coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out)
     ket_pairs(bs);
     std::fill(out, out + (size_t)nbf * nbf, 0.0);
     const int maxlaux = max_aux_l(aux, 0, (int)aux->shells.size());
+    const std::vector<std::vector<int>> batches = aux_batches(aux, 0, (int)aux->shells.size());
 #pragma omp parallel num_threads(nthreads())
     {
         Scratch3c w(maxlaux, bs->maxl);
@@ -972,12 +973,23 @@ int coulomb_3c_sum(const f2ints_basis* aux, const f2ints_basis* bs, double* out)
             const int nsa = nsph(bs->shells[f.a0].l), nsbb = nsph(bs->shells[f.b0].l), nab = nsa * nsbb;
             const size_t ncp = f.cols.size();
             sum.assign(ncp * nab, 0.0);
-            for (const Shell& P : aux->shells) {
+            auto add = [&](const Shell& P, const std::vector<double>& acc) {
                 const int nsP = nsph(P.l);
-                eval_fam(bs, f, P, w);
                 for (size_t c = 0; c < ncp; ++c)
                     for (int sP = 0; sP < nsP; ++sP)
-                        for (int i = 0; i < nab; ++i) sum[c * nab + i] += w.acc[(c * nsP + sP) * nab + i];
+                        for (int i = 0; i < nab; ++i) sum[c * nab + i] += acc[(c * nsP + sP) * nab + i];
+            };
+            // aux shells in batch order (by l, then index)
+            for (const std::vector<int>& bt : batches) {
+                if (bt.size() == (size_t)kLanes) {
+                    const Shell* Ps[kLanes];
+                    for (int l = 0; l < kLanes; ++l) Ps[l] = &aux->shells[bt[l]];
+                    eval_fam4(bs, f, Ps, w);
+                    for (int l = 0; l < kLanes; ++l) add(*Ps[l], w.acc4[l]);
+                } else {
+                    eval_fam(bs, f, aux->shells[bt[0]], w);
+                    add(aux->shells[bt[0]], w.acc);
+                }
             }
             // this family pair owns these entries: no other thread writes them
             for (size_t c = 0; c < ncp; ++c) {
