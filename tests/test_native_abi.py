@@ -49,3 +49,14 @@ def test_f2probe_exports_every_declared_symbol():
     assert lib.f2probe_read(None, 16, 1, None, 0, None, ctypes.byref(ms)) == -1
     assert b"null argument" in lib.f2probe_last_error()
 
+
+def test_f2ints_exports_every_declared_symbol():
+    from forte2_amd import _ints_proto, _native
+
+    lib = _native.ints_lib()
+    names = _declared("f2ints.h")
+    assert "f2ints_coulomb_3c_pairs_aux" in names
+    assert set(names) == set(_ints_proto.F2INTS_SYMBOLS)
+    for n in names:
+        assert hasattr(lib, n), n
+
