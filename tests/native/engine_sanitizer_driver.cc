@@ -110,8 +110,25 @@ int main() {
         maxabs = std::fmax(maxabs, std::fabs(ref));
     }
     if (maxdiff > 1e-12 * maxabs) return fail("coulomb_3c_sum != sum of blocks");
+    // the remaining entry points
+    std::vector<double> eri(N * N * N * N);
+    if (f2ints_coulomb_4c(bs, eri.data())) return fail("coulomb_4c");
+    for (int64_t i = 0; i < N; ++i)
+        for (int64_t j = 0; j < N; ++j)
+            if (std::fabs(eri[((i * N + j) * N + j) * N + i] - eri[((j * N + i) * N + i) * N + j]) > 1e-13)
+                return fail("4c symmetry");
+    double F[32];
+    if (f2ints_boys(10, 3.7, F) || !(F[0] > F[10] && F[10] > 0.0)) return fail("boys");
+    const int lmax = f2ints_max_l();
+    std::vector<double> T3((size_t)(lmax + 1) * (lmax + 2) / 2 * (2 * lmax + 1));
+    if (lmax < 4 || f2ints_cart2sph(lmax, T3.data())) return fail("cart2sph");
+    if (f2ints_set_threads(2)) return fail("set_threads");
+    std::vector<double> sq2(Q * N * N);
+    if (f2ints_coulomb_3c(ax, bs, 0, nsh_aux, sq2.data()) || std::memcmp(sq2.data(), sq.data(), sizeof(double) * sq.size()))
+        return fail("coulomb_3c with 2 threads");
     // error paths
     if (f2ints_coulomb_3c_pairs_aux(ax, bs, 4, 2, 0, npairs, packed.data(), nc) == 0) return fail("bad range accepted");
+    if (f2ints_boys(-1, 1.0, F) == 0) return fail("bad Boys order accepted");
     f2ints_basis_destroy(bs);
     f2ints_basis_destroy(ax);
     std::printf("ok N=%lld Q=%lld packed=%lld\n", (long long)N, (long long)Q, (long long)nc);
