@@ -14,7 +14,13 @@ Printed: one JSON line with metric/value/unit, the roofline of the dominant
 kernel (K half-transform, measured with HIP events on the launch stream), the
 achievable MFMA and HBM-read ceilings measured on the device by peak probes
 beside the published ones, and the CPU baseline (the numpy oracle on a bounded
-aux-slice sample, scaled).
+aux-slice sample, scaled).  After the timed builds the synthetic B leaves HBM
+and two more legs of the metric run (outside the timed region):
+  * accuracy: the real (H2O)_64 RHF (seed 64, 14.5 A cube) in FP64 and in the
+    precision form; ``final_energy_error_vs_fp64_Eh`` (bar: 1e-8 Eh);
+  * latency: the first 48 requests of the seeded (H2O)_n serving stream
+    (n ~ U{1..24}) at 2 requests/s through the continuous-batching server;
+    ``latency_p50_s`` / ``latency_p99_s`` and jobs/s.
 """
 
 import argparse
@@ -28,12 +34,44 @@ sys.path.insert(0, ROOT)
 
 METRIC = "JK builds/s + MFMA util, (H2O)_64 mixed vs FP64 at 1e-8 Eh; p99 SCF latency"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
-# HBM bytes per launch of the half-transform at the default workload, measured
-# with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a plain copy of known size:
-# x2 and x1) on these kernels and launch shapes (profiles/pmc_traffic_h2o64_bf16x3.md);
-# other workloads report null.
-MEASURED_KX_TRAFFIC = {(64, "bf16x3"): 18.72e9 + 3.62e9}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per kernel launch measured by `bench.py --measure-traffic` (rocprofv3
+# FETCH_SIZE / WRITE_SIZE, each scaled by what it reads for a plain device copy
+# of known size), keyed by workload and precision, together with the SHA-256 of
+# the kernel sources it was measured on: a table measured on other sources is
+# not reported (traffic null).
+TRAFFIC_TABLE = os.path.join(ROOT, "forte2_amd", "data", "traffic_gfx950.json")
+KERNEL_SOURCES = ("forte2_amd/csrc/f2jk.hip", "forte2_amd/csrc/f2jk_kernels.h", "forte2_amd/csrc/f2jk_group.h")
+
+
+def kernel_sources_sha256():
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def measured_traffic(waters, precision, kernel_prefix):
+    """(HBM bytes per launch of the first kernel whose name starts with
+    kernel_prefix, note) from the traffic table, or (None, reason)."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None, "no traffic table"
+    if tab.get("kernel_sources_sha256") != kernel_sources_sha256():
+        return None, "traffic table measured on other kernel sources (rerun bench.py --measure-traffic)"
+    ent = tab.get("entries", {}).get(f"{waters}:{precision}", {})
+    for name, k in ent.get("kernels", {}).items():
+        if name.startswith(kernel_prefix):
+            return k["read_bytes"] + k["write_bytes"], (
+                f"{name}: {k['read_bytes']:.4g} B read + {k['write_bytes']:.4g} B written per launch "
+                f"(rocprofv3 FETCH_SIZE x{tab['fetch_factor']:.3g}, WRITE_SIZE x{tab['write_factor']:.3g}, "
+                f"factors from a plain {tab['copy_bytes']} B copy)")
+    return None, f"no entry for {waters}:{precision}"
 
 
 def parse():
@@ -46,7 +84,19 @@ def parse():
     ap.add_argument("--cpu-sample-aux", type=int, default=384,
                     help="aux functions in the CPU baseline sample (0 = skip)")
     ap.add_argument("--profile-steps", action="store_true",
-                    help="run only the timed mixed steps (for rocprofv3)")
+                    help="run only the timed steps of one precision (for rocprofv3)")
+    ap.add_argument("--profile-precision", default=None, choices=["fp64", "fp32", "bf16x3"],
+                    help="precision of --profile-steps (default: --mixed)")
+    ap.add_argument("--measure-traffic", action="store_true",
+                    help="run rocprofv3 FETCH_SIZE / WRITE_SIZE passes over --profile-steps and a plain "
+                         "copy as child processes and write the per-launch HBM traffic table")
+    ap.add_argument("--traffic-out", default=None, help="output of --measure-traffic (default: the package table)")
+    ap.add_argument("--scf-leg", type=int, default=1,
+                    help="1: also run the real (H2O)_n RHF in FP64 and in the precision form after the "
+                         "microbenchmark and report the final-energy error (0: skip)")
+    ap.add_argument("--serve-leg-requests", type=int, default=48,
+                    help="requests of the serving leg of the default run (0: skip)")
+    ap.add_argument("--serve-leg-rate", type=float, default=2.0, help="arrival rate of the serving leg (1/s)")
     ap.add_argument("--scf", action="store_true",
                     help="instead of the J/K microbenchmark, run a real (H2O)_n RHF in FP64 and in the "
                          "precision form and report energies, errors and SCF latency")
@@ -129,19 +179,23 @@ def measured_peaks(read_buf=None, read_reps=3):
     return out
 
 
-def scf_mode(args):
+def scf_leg(n, mixed, seed=None, log=None):
     """Real (H2O)_n cc-pVDZ/cc-pVTZ-JKFIT RHF: integrals on the host streamed into
-    HBM, FP64 plain form and the precision form on the same B."""
+    HBM, then the FP64 plain form and the precision form on the same B, both
+    to e_tol 1e-10.  Returns energies, iterations, SCF seconds and the final
+    energy error of the precision form against FP64."""
     import torch
 
     from forte2_amd import geometry as geo
+    from forte2_amd import integrals
     from forte2_amd.scf import RHF
     from forte2_amd.system import System
 
+    log = log or (lambda msg: None)
     dev = torch.device("cuda", 0)
-    n = args.waters
     box = {16: 9.0, 64: 14.5}.get(n, 3.6 * n ** (1.0 / 3.0))
-    seed = n if args.seed is None else args.seed
+    seed = n if seed is None else seed
+    integrals.set_threads(host_cores())
     t0 = time.perf_counter()
     s = System(geo.water_cluster(n, box, seed=seed), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
     fb = s.fock_builder
@@ -150,18 +204,18 @@ def scf_mode(args):
     def progress(msg):
         if time.perf_counter() - last[0] > 5.0:
             last[0] = time.perf_counter()
-            print(f"[scf] {msg}", file=sys.stderr, flush=True)
+            log(msg)
 
     fb.progress = progress
     fb.B_Pmn
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
-    print(f"[scf] (H2O)_{n} N={s.nbf} Q={s.naux}: B formed in {t_b:.1f} s", file=sys.stderr, flush=True)
+    log(f"(H2O)_{n} N={s.nbf} Q={s.naux}: B formed in {t_b:.1f} s")
     out = {"workload": f"(H2O)_{n} RHF cc-pVDZ/cc-pVTZ-JKFIT, box {box} A, seed {seed}",
            "nbf": s.nbf, "naux": s.naux, "nocc": s.Zsum // 2, "B_formation_s": t_b,
            "B_formation_detail": fb.B_stats, "e_tol": 1e-10}
     runs = {}
-    for prec in ("fp64", args.mixed):
+    for prec in ("fp64", mixed):
         fb.timing = False
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -170,10 +224,21 @@ def scf_mode(args):
         dt = time.perf_counter() - t1
         stages = "".join(h[1][0] for h in r.history)
         runs[prec] = {"E": r.E, "iterations": r.iter, "scf_seconds": dt, "stages": stages,
-                      "lp_error_vs_fp64_Eh": r.lp_error_vs_fp64}
-        print(f"[scf] {prec}: E={r.E:.12f} iters={r.iter} {dt:.2f} s stages={stages}", file=sys.stderr, flush=True)
+                      "converged": bool(r.converged), "lp_error_vs_fp64_Eh": r.lp_error_vs_fp64}
+        log(f"{prec}: E={r.E:.12f} iters={r.iter} {dt:.2f} s stages={stages}")
     out["runs"] = runs
-    out["final_energy_error_vs_fp64_Eh"] = abs(runs[args.mixed]["E"] - runs["fp64"]["E"])
+    out["final_energy_error_vs_fp64_Eh"] = abs(runs[mixed]["E"] - runs["fp64"]["E"])
+    out["scf_speedup_vs_fp64"] = runs["fp64"]["scf_seconds"] / runs[mixed]["scf_seconds"]
+    fb.release()
+    s.fock_builder = None
+    del s, fb
+    torch.cuda.empty_cache()
+    return out
+
+
+def scf_mode(args):
+    out = scf_leg(args.waters, args.mixed, args.seed,
+                  log=lambda m: print(f"[scf] {m}", file=sys.stderr, flush=True))
     print(json.dumps(out))
 
 
@@ -185,6 +250,33 @@ def host_cores():
         return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
     except ValueError:
         return n
+
+
+def serve_leg(nreq, rate, nmax=24, precision="fp64", slots=12, prep_threads=2, log=None):
+    """A bounded slice of the serving workload: the first nreq requests of the
+    seeded (H2O)_n stream (n ~ U{1..nmax}, seed 7) at Poisson rate `rate`,
+    served with continuous batching; p50/p99 latency and jobs/s."""
+    import torch
+
+    from forte2_amd import integrals
+    from forte2_amd.serving import SCFServer, serving_stream
+
+    integrals.set_threads(host_cores())
+    reqs = serving_stream(nreq=nreq, rate=rate, nmax=nmax)
+    srv = SCFServer(max_active=slots, prep_threads=prep_threads, precision=precision, log=log)
+    res = srv.run(reqs)
+    torch.cuda.synchronize()
+    s = srv.summary()
+    return {
+        "workload": f"{len(reqs)} RHF requests, (H2O)_n n~U{{1..{nmax}}}, cc-pVDZ/cc-pVTZ-JKFIT, "
+                    f"Poisson arrivals at {rate}/s (seed 7), {slots} slots, {precision}",
+        "rate_per_s": rate, "requests": s["requests"], "completed": s["completed"],
+        "jobs_per_s": s["jobs_per_s"], "latency_p50_s": s["latency_p50_s"],
+        "latency_p99_s": s["latency_p99_s"], "latency_mean_s": s["latency_mean_s"],
+        "mean_group_size": s["mean_group_size"], "wall_s": s["wall_s"],
+        "largest_nbf": max((r.nbf for r in res.values()), default=0),
+        "group_failures": s["group_failures"], "build_failures": s["build_failures"],
+    }
 
 
 def serve_mode(args):
@@ -316,6 +408,75 @@ def otf_mode(args):
     }))
 
 
+def measure_traffic(args):
+    """HBM bytes per kernel launch from rocprofv3 counters, one counter per pass,
+    each pass a child process: FETCH_SIZE and WRITE_SIZE over `--profile-steps`
+    (2 builds of each precision at the default workload) and over a plain
+    4 GiB device copy, whose known bytes give the factors that turn the counters
+    into bytes.  Writes the table bench.py reads (with the SHA-256 of the kernel
+    sources it was measured on).  This process itself never touches the GPU."""
+    import csv
+    import glob
+    import subprocess
+    import tempfile
+
+    work = tempfile.mkdtemp(prefix="f2jk_pmc_")
+    commands = []
+
+    def pass_(counter, tag, bench_args, regex=None):
+        d = os.path.join(work, tag)
+        cmd = ["rocprofv3", "--pmc", counter]
+        if regex:
+            cmd += ["--kernel-include-regex", regex]
+        cmd += ["-f", "csv", "-d", d, "-o", "run", "--", sys.executable, "-u", os.path.join(ROOT, "bench.py")]
+        cmd += bench_args
+        commands.append(" ".join(cmd[:cmd.index("--") + 1] + ["python3", "-u", "bench.py"] + bench_args))
+        print(f"[traffic] {counter} {tag}", file=sys.stderr, flush=True)
+        subprocess.run(cmd, check=True, timeout=900, cwd=ROOT)
+        per = {}
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row["Kernel_Name"]
+                    name = name[5:] if name.startswith("void ") else name
+                    name = name.split("(")[0]
+                    disp = row.get("Dispatch_Id") or row.get("Correlation_Id")
+                    per.setdefault(name, {}).setdefault(disp, 0.0)
+                    per[name][disp] += float(row["Counter_Value"])
+        if not per:
+            raise RuntimeError(f"no {counter} rows under {d}")
+        return {k: (sum(v.values()) / len(v), len(v)) for k, v in per.items()}
+
+    copy_bytes = 4 << 30
+    fc = pass_("FETCH_SIZE", "fetch_copy", ["--copy-calibration"])
+    wc = pass_("WRITE_SIZE", "write_copy", ["--copy-calibration"])
+    copy_name = max(fc, key=lambda k: fc[k][0])
+    fetch_factor = copy_bytes / 1024.0 / fc[copy_name][0]
+    write_factor = copy_bytes / 1024.0 / wc[copy_name][0]
+    entries = {}
+    for prec in ("fp64", args.mixed):
+        run = ["--profile-steps", "--profile-precision", prec, "--steps", "2", "--warmup", "0",
+               "--waters", str(args.waters)]
+        fe = pass_("FETCH_SIZE", f"fetch_{prec}", run, regex="f2jk")
+        wr = pass_("WRITE_SIZE", f"write_{prec}", run, regex="f2jk")
+        kern = {}
+        for name in sorted(set(fe) | set(wr)):
+            f_kib, nl = fe.get(name, (0.0, 0))
+            w_kib, _ = wr.get(name, (0.0, 0))
+            kern[name] = {"launches": nl, "fetch_kib": f_kib, "write_kib": w_kib,
+                          "read_bytes": f_kib * 1024.0 * fetch_factor, "write_bytes": w_kib * 1024.0 * write_factor}
+        entries[f"{args.waters}:{prec}"] = {"kernels": kern}
+    tab = {"kernel_sources_sha256": kernel_sources_sha256(), "fetch_factor": fetch_factor,
+           "write_factor": write_factor, "copy_bytes": copy_bytes, "copy_kernel": copy_name,
+           "commands": commands, "entries": entries}
+    out = args.traffic_out or TRAFFIC_TABLE
+    with open(out, "w") as f:
+        json.dump(tab, f, indent=1, sort_keys=True)
+    print(json.dumps({"traffic_table": out, "fetch_factor": fetch_factor, "write_factor": write_factor}))
+
+
 def timed_builds(fb, C, prec, steps, warmup):
     import torch
 
@@ -421,6 +582,9 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.measure_traffic:
+        measure_traffic(args)
+        return
     if args.peaks:
         print(json.dumps({"measured_peaks": measured_peaks()}))
         return
@@ -439,7 +603,7 @@ def main():
     print(f"[bench] (H2O)_{args.waters}: N={N} Q={Q} o={o} inputs resident", file=sys.stderr, flush=True)
 
     if args.profile_steps:
-        timed_builds(fb, C, args.mixed, args.steps, args.warmup)
+        timed_builds(fb, C, args.profile_precision or args.mixed, args.steps, args.warmup)
         return
 
     t_mix, Jm, Km = timed_builds(fb, C, args.mixed, args.steps, args.warmup)
@@ -476,6 +640,30 @@ def main():
         cpu = cpu_baseline(B, C, min(args.cpu_sample_aux, Q))
         print(f"[bench] cpu baseline: {cpu['value']:.4f} builds/s on {cpu['cores']} cores",
               file=sys.stderr, flush=True)
+    kx_prefix = "f2jk::kx_bf16x3_kernel" if args.mixed == "bf16x3" else "f2jk::kx_kernel<float"
+    traffic, traffic_note = measured_traffic(args.waters, args.mixed, kx_prefix)
+
+    # the synthetic B leaves HBM before the real-integral legs
+    fb.release()
+    del fb, B, C, Jm, Km, J64, K64
+    torch.cuda.empty_cache()
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    scf = None
+    if args.scf_leg:
+        scf = scf_leg(args.waters, args.mixed, log=log)
+        log(f"scf leg: |E_mixed - E_fp64| = {scf['final_energy_error_vs_fp64_Eh']:.2e} Eh")
+    serve = None
+    if args.serve_leg_requests > 0:
+        last = [0.0]
+
+        def slog(m):
+            if time.perf_counter() - last[0] >= 10.0:
+                last[0] = time.perf_counter()
+                log(m)
+
+        serve = serve_leg(args.serve_leg_requests, args.serve_leg_rate, log=slog)
+        log(f"serving leg: p50 {serve['latency_p50_s']:.2f} s p99 {serve['latency_p99_s']:.2f} s "
+            f"{serve['jobs_per_s']:.2f} jobs/s")
 
     line = {
         "metric": METRIC,
@@ -523,9 +711,8 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": MEASURED_KX_TRAFFIC.get((args.waters, args.mixed)),
-            "traffic_note": "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (copy-calibrated); "
-                            "algorithmic bytes per launch %.3g" % kx_bytes_launch,
+            "traffic": traffic,
+            "traffic_note": traffic_note + "; algorithmic bytes per launch %.4g" % kx_bytes_launch,
             "launches_per_build": la_mix["kx"],
             "avg_launch_ms": kx_ms_launch,
             # BF16x3 runs three bf16 MFMA products per algorithmic product
@@ -538,6 +725,14 @@ def main():
         },
         "measured_peaks": peaks,
         "cpu_baseline": cpu,
+        # the accuracy leg of the metric: the real (H2O)_n RHF in the precision
+        # form against FP64 (final-energy error; the bar is 1e-8 Eh)
+        "final_energy_error_vs_fp64_Eh": scf["final_energy_error_vs_fp64_Eh"] if scf else None,
+        "scf": scf,
+        # the latency leg: p50/p99 of a bounded slice of the serving stream
+        "latency_p50_s": serve["latency_p50_s"] if serve else None,
+        "latency_p99_s": serve["latency_p99_s"] if serve else None,
+        "serving": serve,
     }
     print(json.dumps(line))
 

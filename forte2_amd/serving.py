@@ -139,7 +139,16 @@ class SCFServer:
         self.log = log
         self.results = {}
         self.ticks = 0
+        self.wall = 0.0
         self.group_sizes = []
+        # grouped builds the library declined (ValueError / MemoryError /
+        # out of HBM: the jobs continue alone), grouped builds that failed
+        # otherwise (a HIP launch or runtime error, or a bug: the jobs of the
+        # group end with the error recorded) and single-job builds that failed
+        # in the library or the HIP runtime
+        self.group_declines = 0
+        self.group_failures = 0
+        self.build_failures = 0
 
     # ------------------------------------------------------------------ host preparation
     @staticmethod
@@ -259,10 +268,22 @@ class SCFServer:
                 grp = jobs[g0:g0 + MAX_GROUP]
                 try:
                     outs = self._build_group(grp)
-                except Exception:  # the library declines or fails the group: these jobs run alone from now on
+                except (ValueError, MemoryError, torch.OutOfMemoryError):
+                    # the library declines the group (shape it does not take,
+                    # workspace it cannot get): these jobs run alone from now on
+                    self.group_declines += 1
                     for jb in grp:
                         jb._groupable[(kind, prec, nd)] = False
                     solo.extend(grp)
+                    continue
+                except Exception as e:
+                    # a launch or runtime failure (NativeError) or a fault of
+                    # the group path itself: the work of these jobs is lost, so
+                    # they end with the error; nothing is retried alone
+                    self.group_failures += 1
+                    for jb in grp:
+                        jb.error = repr(e)
+                        failed.add(id(jb))
                     continue
                 for jb, out in zip(grp, outs):
                     results[id(jb)] = out
@@ -270,6 +291,11 @@ class SCFServer:
         for jb in solo:
             try:
                 results[id(jb)] = jb.scf._execute(jb.pending)
+            except nat.NativeError as e:  # a launch or runtime failure: this job ends, counted
+                self.build_failures += 1
+                jb.error = repr(e)
+                failed.add(id(jb))
+                continue
             except Exception as e:  # this job's build failed: it ends, the others go on
                 jb.error = repr(e)
                 failed.add(id(jb))
@@ -419,6 +445,9 @@ class SCFServer:
             "latency_mean_s": float(lat.mean()),
             "ticks": self.ticks,
             "mean_group_size": float(np.mean(self.group_sizes)) if self.group_sizes else 0.0,
+            "group_declines": self.group_declines,
+            "group_failures": self.group_failures,
+            "build_failures": self.build_failures,
         }
 
 

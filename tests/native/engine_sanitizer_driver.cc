@@ -117,6 +117,15 @@ int main() {
         for (int64_t j = 0; j < N; ++j)
             if (std::fabs(eri[((i * N + j) * N + j) * N + i] - eri[((j * N + i) * N + i) * N + j]) > 1e-13)
                 return fail("4c symmetry");
+    // E_nn = sum_{a<b} Z_a Z_b / R_ab, summed here by hand over the three centres
+    double enn = 0.0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < a; ++b) {
+            const double dx = centres[3 * a] - centres[3 * b], dy = centres[3 * a + 1] - centres[3 * b + 1],
+                         dz = centres[3 * a + 2] - centres[3 * b + 2];
+            enn += Z[a] * Z[b] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    if (std::fabs(f2ints_nuclear_repulsion(3, Z, centres) - enn) > 1e-13 * enn) return fail("nuclear_repulsion");
     double F[32];
     if (f2ints_boys(10, 3.7, F) || !(F[0] > F[10] && F[10] > 0.0)) return fail("boys");
     const int lmax = f2ints_max_l();

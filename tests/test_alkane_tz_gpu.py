@@ -1,0 +1,89 @@
+"""The mixed-size UHF workload on the GPU: all-trans alkanes C_nH_(2n+2), neutral
+(UHF, ms = 0) and cation (UHF doublet, ms = 1/2; two densities per J/K build),
+in cc-pVTZ with the cc-pVTZ-JKFIT fit set (nbf 58n + 28, naux 139n + 60).
+
+* Plain form: the device FP64 UHF energy and <S^2> equal the SCF oracle's
+  (oracle/scf.py, forte2's UHF loop restated) on the same integrals,
+  energy to <= 1e-10 Eh (forte2/scf/uhf.py:61-127).
+* Precision form: the BF16x3 UHF ends within 1e-8 Eh of the FP64 UHF.
+* Precision and serving forms combined: a short alkane stream through the
+  continuous-batching server in BF16x3, grouped builds in use; each job's
+  energy is bitwise the same alone and in the mix, and within 1e-8 Eh of
+  its FP64 energy.
+"""
+
+import pytest
+import torch
+
+from forte2_amd import geometry as geo
+from forte2_amd.scf import UHF
+from forte2_amd.serving import SCFRequest, SCFServer, alkane_stream
+from forte2_amd.system import System
+from oracle import jk as ojk
+from oracle import scf as oscf
+
+pytestmark = pytest.mark.gpu
+
+BASIS, AUX = "cc-pvtz", "cc-pvtz-jkfit"
+
+
+def _system(n, dev):
+    s = System(geo.alkane(n), BASIS, AUX, device=dev)
+    assert (s.nbf, s.naux) == (58 * n + 28, 139 * n + 60)
+    return s
+
+
+@pytest.mark.parametrize("n,charge", [(2, 1), (3, 1), (4, 1), (2, 0), (4, 0)])
+def test_uhf_alkane_tz_plain_form_matches_oracle(dev, n, charge):
+    ms = 0.5 if charge else 0.0
+    print(f"[alkane-gpu] C{n}H{2 * n + 2}{'+' if charge else ''} UHF ms={ms} cc-pVTZ/cc-pVTZ-JKFIT", flush=True)
+    s = _system(n, dev)
+    scf = UHF(charge=charge, ms=ms, e_tol=1e-10)(s).run()
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    ref = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - charge, ms,
+                       X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+    print(f"   device {scf.E:.12f} oracle {ref.E:.12f} diff {scf.E - ref.E:.2e} iters {scf.iter}/{ref.iter} "
+          f"S2 {scf.S2:.10f}/{ref.S2:.10f}", flush=True)
+    assert scf.converged
+    assert abs(scf.E - ref.E) <= 1e-10
+    assert abs(scf.S2 - ref.S2) <= 1e-7
+    s.fock_builder.release()
+
+
+@pytest.mark.parametrize("n", [4])
+def test_uhf_alkane_tz_precision_form(dev, n):
+    s = _system(n, dev)
+    e64 = UHF(charge=1, ms=0.5, e_tol=1e-10)(s).run()
+    mix = UHF(charge=1, ms=0.5, e_tol=1e-10, precision="bf16x3")(s).run()
+    stages = "".join(h[1][0] for h in mix.history)
+    print(f"[alkane-gpu] C{n}H{2 * n + 2}+ bf16x3 {mix.E:.12f} fp64 {e64.E:.12f} diff {mix.E - e64.E:.2e} "
+          f"stages {stages}", flush=True)
+    assert mix.converged and abs(mix.E - e64.E) <= 1e-8
+    assert stages[0] == "l" and stages[-1] == "f"
+    s.fock_builder.release()
+
+
+def test_alkane_stream_precision_and_serving_forms(dev):
+    reqs = alkane_stream(nreq=6, rate=50.0, nmin=2, nmax=4, seed=5)
+    assert {r.charge for r in reqs} == {0, 1}
+    srv = SCFServer(device=dev, e_tol=1e-10, precision="bf16x3")
+    mix = srv.run(reqs)
+    summ = srv.summary()
+    print(f"[alkane-gpu] stream: {summ}", flush=True)
+    assert summ["completed"] == len(reqs)
+    assert summ["mean_group_size"] > 1.0
+    assert summ["group_failures"] == 0 and summ["build_failures"] == 0
+    for r in reqs:
+        again = SCFRequest(r.id, r.xyz, r.basis, r.aux, r.charge, r.ms, 0.0)
+        alone = SCFServer(device=dev, e_tol=1e-10, precision="bf16x3", max_active=1,
+                          prep_threads=1).run([again])[r.id]
+        s = System(r.xyz, r.basis, r.aux, device=dev)
+        e64 = UHF(charge=r.charge, ms=r.ms, e_tol=1e-10)(s).run()
+        s.fock_builder.release()
+        m = mix[r.id]
+        print(f"   req {r.id} (nbf {m.nbf}, charge {r.charge}): mix {m.E:.12f} alone {alone.E:.12f} "
+              f"fp64 {e64.E:.12f} |d|={abs(m.E - e64.E):.2e}", flush=True)
+        assert m.converged and alone.converged
+        assert m.E == alone.E  # bitwise
+        assert abs(m.E - e64.E) <= 1e-8
+    torch.cuda.empty_cache()

@@ -1,0 +1,131 @@
+"""GPU parity at the full (H2O)_64 size: nbf 1536, naux 8896, nocc 320
+(cc-pVDZ orbital basis, cc-pVTZ-JKFIT fit set).
+
+* One J/K build on the whole 168 GB B, in FP64 and in the BF16x3 precision
+  form, against the numpy oracle (oracle/jk.py, forte2's FockBuilder.build_JK
+  restated).  J and K are sums over the aux index P, so the oracle runs on
+  P-slices of the same B streamed to the host and its slice results are
+  summed: the full-size result is checked element by element, not sampled.
+  Bars: FP64 max-abs <= 1e-10; BF16x3 relative to max|.| <= 2e-6 (J, FP32 B
+  with FP64 accumulation) and <= 2e-5 (K, see below).
+* Size-independent properties at full size: J and K exactly symmetric;
+  J linear in D; tr(D K) = sum_P ||C^T B_P C||_F^2 >= 0.
+* The real (H2O)_64 RHF (seed 64, 14.5 A cube): the precision form (BF16x3
+  builds, incremental Fock builds, FP64 final cycles) ends within 1e-8 Eh of
+  the FP64 SCF on the same integrals.
+
+BF16x3 K bound.  Each FP32 operand is carried as hi + lo with hi, lo bf16
+(8 significant bits each, unit roundoff u = 2^-8): |x - hi - lo| <= u^2 |x|,
+and the dropped lo*lo term is <= u^2 |a b|, so a product is off by at most
+~3 u^2 = 4.6e-5 of |a b| and a sum of many products with uncorrelated
+rounding stays well under u^2 = 1.5e-5 of the largest |K| element; the bound
+2e-5 sits just above that, so losing one cross term (error ~u = 4e-3) or a
+wider FP32 accumulation window fails.
+"""
+
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from forte2_amd.jkbuilder import FockBuilder
+from forte2_amd.workloads import orthonormal_C, synthetic_B, water_cluster_shape
+from oracle import jk as ojk
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+REL_J_BF16X3 = 2e-6
+REL_K_BF16X3 = 2e-5
+
+
+def _free():
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def _oracle_by_slices(B, Ch, pblk=256):
+    """Oracle J and K of the whole B, summed over P-slices copied to the host."""
+    Q = B.shape[0]
+    Jr = Kr = None
+    t0 = time.perf_counter()
+    for p0 in range(0, Q, pblk):
+        p1 = min(Q, p0 + pblk)
+        Bs = B[p0:p1].cpu().numpy()
+        J, K = ojk.build_JK(Bs, [Ch])
+        Jr = J[0] if Jr is None else Jr + J[0]
+        Kr = K[0] if Kr is None else Kr + K[0]
+        del Bs
+        if (p0 // pblk) % 8 == 0:
+            print(f"   oracle slices: P < {p1} of {Q} ({time.perf_counter() - t0:.0f} s)", flush=True)
+    return Jr, Kr
+
+
+def test_h2o64_builds_match_the_oracle_at_full_size(dev):
+    N, Q, o = water_cluster_shape(64)
+    print(f"[headline] (H2O)_64 shapes N={N} Q={Q} o={o}: synthetic B in HBM", flush=True)
+    B = synthetic_B(Q, N, seed=64, device=dev)
+    C = orthonormal_C(N, o, seed=64, device=dev)
+    fb = FockBuilder(B=B, device=dev)
+    J64, K64 = fb.build_JK([C])
+    Jm, Km = fb.build_JK([C], precision="bf16x3")
+    torch.cuda.synchronize()
+    J64, K64, Jm, Km = J64[0], K64[0], Jm[0], Km[0]
+    print("[headline] device builds done; properties", flush=True)
+
+    # exact symmetry (both triangles are written from one value)
+    for X in (J64, K64, Jm, Km):
+        assert torch.equal(X, X.T)
+    # tr(D K) = sum_P ||C^T B_P C||^2 >= 0, and tr(D J) = sum_P <B_P, D>^2 >= 0
+    D = C @ C.T
+    assert float(torch.sum(D * K64)) > 0.0 and float(torch.sum(D * J64)) > 0.0
+    # J is linear in D: J(D1 + D2) = J(D1) + J(D2) on the same B
+    C1, C2 = C[:, : o // 2], C[:, o // 2:]
+    D1, D2 = C1 @ C1.T, C2 @ C2.T
+    Ja, Jb, Jab = fb.build_J([D1, D2, D1 + D2])
+    lin = float((Jab - Ja - Jb).abs().max())
+    print(f"   |J(D1+D2) - J(D1) - J(D2)| = {lin:.2e}", flush=True)
+    assert lin <= TOL_FP64
+    # J(D) from the density equals J from the occupied orbitals
+    assert float((Jab - J64).abs().max()) <= TOL_FP64
+    del Ja, Jb, Jab, D, D1, D2
+
+    Jr, Kr = _oracle_by_slices(B, C.cpu().numpy())
+    ej = float(np.abs(J64.cpu().numpy() - Jr).max())
+    ek = float(np.abs(K64.cpu().numpy() - Kr).max())
+    rj = float(np.abs(Jm.cpu().numpy() - Jr).max() / np.abs(Jr).max())
+    rk = float(np.abs(Km.cpu().numpy() - Kr).max() / np.abs(Kr).max())
+    print(f"   fp64: |dJ|={ej:.2e} |dK|={ek:.2e} (max|J|={np.abs(Jr).max():.2e}, max|K|={np.abs(Kr).max():.2e}); "
+          f"bf16x3: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
+    assert ej <= TOL_FP64 and ek <= TOL_FP64
+    assert rj <= REL_J_BF16X3 and rk <= REL_K_BF16X3
+    fb.release()
+    del fb, B, C, J64, K64, Jm, Km
+    _free()
+
+
+def test_h2o64_rhf_precision_form_final_energy(dev):
+    from forte2_amd import geometry as geo
+    from forte2_amd.scf import RHF
+    from forte2_amd.system import System
+
+    print("[headline] (H2O)_64 RHF cc-pVDZ/cc-pVTZ-JKFIT (seed 64, 14.5 A): forming B", flush=True)
+    s = System(geo.water_cluster(64, 14.5, seed=64), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    assert (s.nbf, s.naux, s.Zsum // 2) == (1536, 8896, 320)
+    fb = s.fock_builder
+    fb.B_Pmn
+    torch.cuda.synchronize()
+    e64 = RHF(charge=0, e_tol=1e-10)(s).run()
+    print(f"   fp64: E={e64.E:.12f} iters {e64.iter} {e64.time:.2f} s", flush=True)
+    mix = RHF(charge=0, e_tol=1e-10, precision="bf16x3")(s).run()
+    stages = "".join(h[1][0] for h in mix.history)
+    print(f"   bf16x3: E={mix.E:.12f} iters {mix.iter} {mix.time:.2f} s stages {stages} "
+          f"|dE|={abs(mix.E - e64.E):.2e} lp-vs-fp64 {mix.lp_error_vs_fp64:.2e}", flush=True)
+    assert e64.converged and mix.converged
+    assert abs(mix.E - e64.E) <= 1e-8
+    assert stages[0] == "l" and "i" in stages and stages[-1] == "f"
+    fb.release()
+    s.fock_builder = None
+    del s, fb, e64, mix
+    _free()

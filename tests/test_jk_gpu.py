@@ -135,10 +135,13 @@ def test_bf16x3_precision_form(dev, Q, N, noccs, ks_tile):
         rj = np.abs(J[k].cpu().numpy() - Jr[k]).max() / np.abs(Jr[k]).max()
         rk = np.abs(K[k].cpu().numpy() - Kr[k]).max() / np.abs(Kr[k]).max()
         print(f"   bf16x3 tile {ks_tile} dens {k}: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
-        # J streams the FP32 copy of B (FP64 accumulation); K carries ~16
-        # significant bits per operand (hi/lo bf16 pair) with FP32 accumulation
+        # J streams the FP32 copy of B (FP64 accumulation).  K: each operand is
+        # hi + lo with bf16 hi, lo (unit roundoff u = 2^-8), so the operand
+        # residual and the dropped lo*lo product are each <= u^2 = 1.5e-5 of a
+        # product; summed with uncorrelated rounding the error stays below
+        # u^2 of max|K|.  A lost cross term (~u = 4e-3) fails the bound.
         assert rj < 1e-6
-        assert rk < 1e-4
+        assert rk < 2e-5
         assert torch.equal(K[k], K[k].T)
 
 

@@ -68,3 +68,44 @@ def test_canonical_orth_matches_oracle():
     assert Xd.shape == Xo.shape == (10, 6)
     assert np.abs(Xd.T @ S @ Xd - np.eye(6)).max() < 1e-10
     assert np.abs(Xd @ Xd.T - Xo @ Xo.T).max() < 1e-8
+
+
+def _affine_min(E):
+    """argmin ||E c|| subject to sum(c) = 1, by brute force: c = c0 + Z z with
+    c0 = 1/n and Z an orthonormal basis of {v : sum(v) = 0}, then a plain
+    least-squares solve for z (no Lagrange system, no normalisation)."""
+    n = E.shape[1]
+    c0 = np.full(n, 1.0 / n)
+    Z = np.linalg.svd(np.ones((1, n)))[2][1:].T
+    z = np.linalg.lstsq(E @ Z, -E @ c0, rcond=None)[0]
+    return c0 + Z @ z
+
+
+@pytest.mark.parametrize("impl", ["oracle", "device"])
+def test_diis_coefficients_minimise_the_error_on_the_affine_hull(impl):
+    # diis.py:102-125: the CDIIS coefficients sum to one and minimise the norm
+    # of the combined error vector over the stored window (nvec = 8).  The
+    # stored "Fock matrices" are one-hot vectors, so the extrapolated output is
+    # the coefficient vector itself.
+    rng = np.random.default_rng(12)
+    nit, nvec = 13, 8
+    d = oscf.DIIS(1, nvec, 2) if impl == "oracle" else DeviceDIIS(1, nvec, 2)
+    errs = []
+    for k in range(nit):
+        p = np.zeros(nit)
+        p[k] = 1.0
+        e = rng.standard_normal((4, 5)) * (0.7 ** k)
+        errs.append(e.ravel())
+        if impl == "oracle":
+            c = d.update(p, e)
+        else:
+            c = d.update(torch.from_numpy(p), torch.from_numpy(e)).numpy()
+        if k == 0:
+            assert np.array_equal(c, p)  # fewer than diis_min vectors: no extrapolation
+            continue
+        lo = max(0, k + 1 - nvec)
+        assert np.all(c[:lo] == 0.0) and np.all(c[k + 1:] == 0.0)  # only the window contributes
+        w = c[lo:k + 1]
+        assert abs(w.sum() - 1.0) < 1e-12
+        ref = _affine_min(np.array(errs[lo:k + 1]).T)
+        assert np.abs(w - ref).max() < 1e-8 * max(1.0, np.abs(ref).max())

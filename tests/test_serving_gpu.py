@@ -114,6 +114,7 @@ def test_results_same_alone_and_in_the_mix(dev):
     mix = srv.run(reqs)
     print(f"[serving] mix: {srv.summary()}", flush=True)
     assert srv.summary()["mean_group_size"] > 1.0
+    assert srv.group_failures == 0 and srv.build_failures == 0
     for r in reqs:
         a, m = alone[r.id], mix[r.id]
         print(f"   req {r.id}: alone {a.E:.12f} mix {m.E:.12f} iters {a.iterations}/{m.iterations}", flush=True)
@@ -144,6 +145,7 @@ def test_stream_with_arrivals_and_tight_memory(dev):
     summ = srv.summary()
     print(f"[serving] stream: {summ}", flush=True)
     assert summ["completed"] == len(reqs)
+    assert summ["group_failures"] == 0 and summ["build_failures"] == 0
     assert all(r.latency >= 0 for r in res.values())
     assert summ["latency_p99_s"] >= summ["latency_p50_s"]
 
@@ -178,3 +180,32 @@ def test_failures_are_isolated(dev):
     assert "HBM budget" in res[4].error
     alone = SCFServer(device=dev, e_tol=1e-8).run([SCFRequest(3, w2, arrival=0.0)])
     assert res[3].E == alone[3].E
+
+
+def test_water_stream_full_size_range_matches_plain_scf(dev):
+    # the serving workload over its whole size range: (H2O)_n with n from 1 to
+    # 24 (nbf up to 576, naux up to 3336), Poisson arrivals, continuous
+    # batching; every served energy equals the plain device RHF of the same
+    # request to 1e-10 Eh (rhf.py:35-65 semantics, FP64)
+    from forte2_amd.scf import RHF
+    from forte2_amd.system import System
+
+    reqs = serving_stream(nreq=12, rate=50.0, nmax=24, seed=27)
+    sizes = [r.xyz.count("O ") for r in reqs]
+    assert max(sizes) == 24 and min(sizes) == 1
+    srv = SCFServer(device=dev, e_tol=1e-10)
+    res = srv.run(reqs)
+    summ = srv.summary()
+    print(f"[serving] (H2O)_n n={sizes}: {summ}", flush=True)
+    assert summ["completed"] == len(reqs)
+    assert summ["group_failures"] == 0 and summ["build_failures"] == 0
+    assert summ["mean_group_size"] > 1.0
+    assert summ["latency_p99_s"] >= summ["latency_p50_s"] > 0.0
+    for r, n in zip(reqs, sizes):
+        s = System(r.xyz, r.basis, r.aux, device=dev)
+        ref = RHF(0, e_tol=1e-10)(s).run()
+        s.fock_builder.release()
+        got = res[r.id]
+        print(f"   req {r.id} (H2O)_{n}: served {got.E:.12f} plain {ref.E:.12f} diff {got.E - ref.E:.2e}", flush=True)
+        assert got.converged and abs(got.E - ref.E) <= 1e-10
+    torch.cuda.empty_cache()
