@@ -18,6 +18,7 @@ c_voidp_p = ctypes.POINTER(ctypes.c_void_p)
 F2JK_F64, F2JK_F32 = 0, 1
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 F2JK_WANT_J, F2JK_WANT_K, F2JK_TIMING = 1, 2, 16
+F2JK_NOALLOC_SCRATCH_BYTES = 64 << 20
 PHASES = ("prep", "j_gamma", "j_acc", "kx", "ks", "k_reduce", "total")
 
 F2JK_SYMBOLS = (
@@ -28,6 +29,7 @@ F2JK_SYMBOLS = (
     "f2jk_set_tuning",
     "f2jk_workspace_size",
     "f2jk_build_jk",
+    "f2jk_build_jk_noalloc",
     "f2jk_build_j",
     "f2jk_build_k",
     "f2jk_last_timings",
@@ -73,6 +75,7 @@ def jk_lib():
     lib.f2jk_set_tuning.argtypes = [V, ctypes.c_char_p, I64]
     lib.f2jk_workspace_size.argtypes = [V, I, c_int64_p, I, I, ctypes.POINTER(sz)]
     lib.f2jk_build_jk.argtypes = [V, I, c_voidp_p, c_int64_p, c_int64_p, c_voidp_p, c_voidp_p, I, I, V, sz, V]
+    lib.f2jk_build_jk_noalloc.argtypes = [V, I, c_voidp_p, c_int64_p, c_int64_p, c_voidp_p, c_voidp_p, I, I, V]
     lib.f2jk_build_j.argtypes = [V, I, c_voidp_p, I64, c_voidp_p, I, I, V, sz, V]
     lib.f2jk_build_k.argtypes = [V, I, c_voidp_p, c_int64_p, c_int64_p, c_voidp_p, I, I, V, sz, V]
     lib.f2jk_last_timings.argtypes = [V, c_double_p, I]

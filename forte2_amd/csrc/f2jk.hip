@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,10 @@ struct Plan {
     size_t off_cp32 = 0, off_cth = 0, off_ctl = 0;
     size_t off_cp = 0, off_d = 0, off_dpart = 0, off_gpart = 0, off_gamma = 0, off_jpart = 0,
            off_x = 0, off_kpart = 0, total = 0;
+    // sizes of the regions that scale with the launch-shape knobs
+    size_t jpart_bytes = 0, x_bytes = 0, kpart_bytes = 0;
+    // the workspace is the library's scratch array: the build may not allocate
+    bool no_alloc = false;
 };
 
 int j_vec_for(int dtype, int64_t N) {
@@ -124,8 +129,18 @@ int ensure_rowoff(f2jk_ctx* ctx, int vec) {
     return F2JK_OK;
 }
 
+// Limits of a plan that are otherwise the context's launch-shape knobs; the
+// no-alloc entry point lowers them until the plan fits its scratch array.
+struct PlanCaps {
+    size_t xchunk_bytes;
+    int target_blocks;
+};
+
 int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags,
-              bool j_from_c, Plan& p, int jvec_override = 0, bool grouped = false) {
+              bool j_from_c, Plan& p, int jvec_override = 0, bool grouped = false,
+              const PlanCaps* caps = nullptr) {
+    const size_t xchunk_bytes = caps ? caps->xchunk_bytes : ctx->xchunk_bytes;
+    const int target_blocks = caps ? caps->target_blocks : ctx->target_blocks;
     if (ctx->nbf <= 0 || ctx->naux <= 0) return fail(F2JK_ESTATE, "B has not been set");
     if (ndens < 1 || ndens > kMaxDens)
         return fail(F2JK_EINVAL, "ndens must be in [1, " + std::to_string(kMaxDens) + "]");
@@ -180,7 +195,7 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         p.nvec = (int)nvec;
         p.neg = (int)cdiv(nvec, (int64_t)kJThreads * kEPT);
         p.npc1 = (int)cdiv(Q, kJ1PerBlock);
-        p.npc2 = (int)std::max<int64_t>(1, std::min<int64_t>(Q, cdiv(ctx->target_blocks, p.neg)));
+        p.npc2 = (int)std::max<int64_t>(1, std::min<int64_t>(Q, cdiv(target_blocks, p.neg)));
         p.ppw2 = (int)cdiv(Q, p.npc2);
         p.npc2 = (int)cdiv(Q, p.ppw2);
         if (j_from_c) {
@@ -194,11 +209,12 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         p.off_gamma = off;
         off += align_up((size_t)ndens * Q * 8);
         p.off_jpart = off;
-        off += align_up((size_t)p.npc2 * ndens * (size_t)nvec * p.jvec * 8);
+        p.jpart_bytes = align_up((size_t)p.npc2 * ndens * (size_t)nvec * p.jvec * 8);
+        off += p.jpart_bytes;
     }
     if (p.want_k) {
         const size_t per_p = (size_t)N * p.OPAD * p.tk;
-        int64_t pc = (int64_t)(ctx->xchunk_bytes / std::max<size_t>(per_p, 1));
+        int64_t pc = (int64_t)(xchunk_bytes / std::max<size_t>(per_p, 1));
         pc = std::max<int64_t>(1, std::min<int64_t>(pc, Q));
         p.Pc = (int)pc;
         p.nchunks = (int)cdiv(Q, pc);
@@ -209,7 +225,7 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
             p.ks_tile = ctx->ks_tile ? ctx->ks_tile : (N >= 768 ? 256 : 128);
         p.ntile = (int)cdiv(N, p.ks_tile);
         p.nlower = p.ntile * (p.ntile + 1) / 2;
-        const int64_t wg_target = p.ks_tile == 256 ? ctx->target_blocks / 4 : ctx->target_blocks / 2;
+        const int64_t wg_target = p.ks_tile == 256 ? target_blocks / 4 : target_blocks / 2;
         p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(wg_target, p.nlower)));
         if (precision == F2JK_PREC_BF16X3) {
             // the tile width (in 32-column units) within the cap that computes
@@ -232,9 +248,11 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         else
             p.kva = (N % 4 == 0) ? 4 : ((N % 2 == 0) ? 2 : 1);
         p.off_x = off;
-        off += align_up((size_t)p.Pc * per_p);
+        p.x_bytes = align_up((size_t)p.Pc * per_p);
+        off += p.x_bytes;
         p.off_kpart = off;
-        off += align_up((size_t)p.nsplit * ndens * N * N * 8);
+        p.kpart_bytes = align_up((size_t)p.nsplit * ndens * N * N * 8);
+        off += p.kpart_bytes;
     }
     p.total = std::max<size_t>(off, 256);
     return F2JK_OK;
@@ -324,6 +342,8 @@ int launch_j_nd(f2jk_ctx* ctx, const Plan& p, const TB* B, int nd, const double*
 
 int run_j(f2jk_ctx* ctx, const Plan& p, const double* const* D, int64_t ldd, double* const* J,
           char* ws, hipStream_t st) {
+    if (p.no_alloc && !ctx->rowoff[p.jvec])
+        return fail(F2JK_ESTATE, "J element table missing (set B of this precision first)");
     int rc = ensure_rowoff(ctx, p.jvec);
     if (rc) return rc;
     if (p.prec == F2JK_PREC_FP64) {
@@ -522,11 +542,12 @@ int check_ws(const Plan& p, void* ws, size_t ws_bytes) {
 int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t* nocc,
                  const int64_t* ldc, const double* const* Din, int64_t ldd, double* const* J,
                  double* const* K, int precision, int flags, void* ws, size_t ws_bytes, void* stream,
-                 bool j_from_c) {
+                 bool j_from_c, const PlanCaps* caps = nullptr, bool no_alloc = false) {
     if (!ctx) return fail(F2JK_EINVAL, "null context");
     Plan p;
-    int rc = make_plan(ctx, ndens, nocc, precision, flags, j_from_c, p);
+    int rc = make_plan(ctx, ndens, nocc, precision, flags, j_from_c, p, 0, false, caps);
     if (rc) return rc;
+    p.no_alloc = no_alloc;
     if ((rc = check_ws(p, ws, ws_bytes))) return rc;
     for (int d = 0; d < ndens; ++d) {
         if (p.want_j && !J[d]) return fail(F2JK_EINVAL, "J output pointer is NULL");
@@ -576,6 +597,57 @@ int build_common(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t
     TIM(F2JK_PH_TOTAL, false);
     g_last_error.clear();
     return F2JK_OK;
+}
+
+// --------------------------------------------------------------------------- no-alloc builds
+// Scratch of f2jk_build_jk_noalloc: one array per device, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the
+// device memory; the device address comes from hipGetSymbolAddress, once per
+// device (f2jk_ctx_create), and only that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[F2JK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// with `device` current
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(F2JK_EUNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return F2JK_OK;
+    void* ptr = nullptr;
+    F2JK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr || ((uintptr_t)ptr) % 256) return fail(F2JK_EHIP, "scratch array has no aligned device address");
+    g_scratch_ptr[device] = ptr;
+    return F2JK_OK;
+}
+
+// The widest launch shapes whose build_jk plan fits `budget` bytes: starting
+// from the context's knobs, the X chunk or the split count (the larger of the
+// two regions first) is halved until the plan fits; ENOMEM when even one P
+// per chunk and one split do not.
+int fit_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision, int flags, size_t budget,
+             PlanCaps& caps) {
+    caps = PlanCaps{ctx->xchunk_bytes, ctx->target_blocks};
+    for (int it = 0; it < 256; ++it) {
+        Plan p;
+        int rc = make_plan(ctx, ndens, nocc, precision, flags, true, p, 0, false, &caps);
+        if (rc) return rc;
+        if (p.total <= budget) return F2JK_OK;
+        const size_t per_p = p.want_k ? (size_t)p.N * p.OPAD * p.tk : 0;
+        if (p.want_k && p.Pc > 1 && p.x_bytes >= p.jpart_bytes + p.kpart_bytes)
+            caps.xchunk_bytes = per_p * (size_t)(p.Pc / 2);
+        else if (caps.target_blocks > 1)
+            caps.target_blocks /= 2;
+        else if (p.want_k && p.Pc > 1)
+            caps.xchunk_bytes = per_p * (size_t)(p.Pc / 2);
+        else
+            return fail(F2JK_ENOMEM, "build needs " + std::to_string(p.total) +
+                                         " bytes of scratch at its smallest launch shapes; the no-alloc scratch "
+                                         "array holds " + std::to_string(budget) + " (use f2jk_build_jk with a "
+                                         "caller workspace)");
+    }
+    return fail(F2JK_ENOMEM, "no launch shape fits the no-alloc scratch array");
 }
 
 // --------------------------------------------------------------------------- grouped builds
@@ -672,6 +744,8 @@ int f2jk_ctx_create(int device, f2jk_ctx** out) {
     F2JK_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(F2JK_EINVAL, "device index out of range");
     F2JK_HIP(hipSetDevice(device));
+    int rc = scratch_init(device);
+    if (rc) return rc;
     f2jk_ctx* ctx = new f2jk_ctx();
     ctx->device = device;
     for (int i = 0; i < kMaxEvents + 2; ++i) {
@@ -794,6 +868,20 @@ int f2jk_build_jk(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_
     if (!C || !nocc) return fail(F2JK_EINVAL, "C and nocc are required");
     return build_common(ctx, ndens, C, nocc, ldc, nullptr, 0, J, K, precision, flags, ws, ws_bytes,
                         stream, true);
+}
+
+int f2jk_build_jk_noalloc(f2jk_ctx* ctx, int ndens, const double* const* C, const int64_t* nocc,
+                          const int64_t* ldc, double* const* J, double* const* K, int precision, int flags,
+                          void* stream) {
+    if (!ctx) return fail(F2JK_EINVAL, "null context");
+    if (!C || !nocc) return fail(F2JK_EINVAL, "C and nocc are required");
+    void* ws = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!ws) return fail(F2JK_ESTATE, "no scratch address for this device");
+    PlanCaps caps;
+    int rc = fit_plan(ctx, ndens, nocc, precision, flags, F2JK_NOALLOC_SCRATCH_BYTES, caps);
+    if (rc) return rc;
+    return build_common(ctx, ndens, C, nocc, ldc, nullptr, 0, J, K, precision, flags, ws,
+                        F2JK_NOALLOC_SCRATCH_BYTES, stream, true, &caps, true);
 }
 
 int f2jk_build_j(f2jk_ctx* ctx, int ndens, const double* const* D, int64_t ldd, double* const* J,

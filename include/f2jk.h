@@ -116,6 +116,29 @@ int f2jk_build_jk(f2jk_ctx* ctx, int ndens, const double* const* C,
                   double* const* K, int precision, int flags, void* ws,
                   size_t ws_bytes, void* stream);
 
+/* Bytes of the scratch array that f2jk_build_jk_noalloc works in (64 MiB per
+ * device, held by the library from the first f2jk_ctx_create on that device
+ * until the process ends). */
+#define F2JK_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* f2jk_build_jk without a caller workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The build works in the library's per-device scratch
+ * array of F2JK_NOALLOC_SCRATCH_BYTES; it calls no hipMalloc/hipFree and no
+ * allocating library routine.  Launch shapes are the context's, lowered (X
+ * chunk, split count) until the build fits the array; a build that does not
+ * fit even at one P per chunk and one split fails with F2JK_ENOMEM before
+ * anything is launched (one density of a cc-pVDZ water cluster fits up to
+ * nbf 1152, (H2O)_48; nbf 1536, (H2O)_64, does not).  Results equal
+ * f2jk_build_jk's to its tolerances, bitwise when the context's own launch
+ * shapes already fit.  Every call on a device shares that device's array:
+ * calls that overlap on one device (other streams, other threads) race, so
+ * callers serialise them. */
+int f2jk_build_jk_noalloc(f2jk_ctx* ctx, int ndens, const double* const* C,
+                          const int64_t* nocc, const int64_t* ldc, double* const* J,
+                          double* const* K, int precision, int flags, void* stream);
+
 /* build_J (forte2/jkbuilder/jkbuilder.py:161-166): J_k = sum_P B_P <B_P, D_k>
  * for symmetric D_k (only the lower triangle of D_k is read). */
 int f2jk_build_j(f2jk_ctx* ctx, int ndens, const double* const* D, int64_t ldd,

@@ -1,0 +1,118 @@
+"""f2jk_build_jk_noalloc (no counterpart in the reference; for callers that
+cannot allocate device memory) against f2jk_build_jk, both through the C ABI.
+
+Two calls in a row on one device share the library's scratch array, the
+second with a smaller input; each result is compared with f2jk_build_jk's on
+the same inputs: FP64 to <= 1e-10 max-abs (the FP64 bar of the J/K builds),
+bitwise where the context's own launch shapes already fit the scratch array,
+and BF16x3 to 2e-5 of max|K| (its split bound) where the fitted launch shapes
+change the FP32 summation.  A build too large for the array is refused with
+F2JK_ENOMEM before anything runs.
+"""
+
+import ctypes
+
+import pytest
+import torch
+
+from forte2_amd import _native as nat
+from forte2_amd.workloads import orthonormal_C, synthetic_B
+
+pytestmark = pytest.mark.gpu
+
+ENOMEM = -3
+
+
+def _ctx(dev, B64, B32=None):
+    lib = nat.jk_lib()
+    h = ctypes.c_void_p()
+    nat.jk_check(lib.f2jk_ctx_create(dev.index, ctypes.byref(h)), "f2jk_ctx_create")
+    Q, N = B64.shape[0], B64.shape[1]
+    nat.jk_check(lib.f2jk_set_B(h, nat.F2JK_F64, ctypes.c_void_p(B64.data_ptr()), Q, N), "set_B")
+    if B32 is not None:
+        nat.jk_check(lib.f2jk_set_B(h, nat.F2JK_F32, ctypes.c_void_p(B32.data_ptr()), Q, N), "set_B32")
+    return h
+
+
+def _outs(n, N, dev):
+    return [torch.empty(N, N, dtype=torch.float64, device=dev) for _ in range(n)]
+
+
+def _build(h, Cs, prec, noalloc, dev):
+    lib = nat.jk_lib()
+    N = Cs[0].shape[0]
+    J, K = _outs(len(Cs), N, dev), _outs(len(Cs), N, dev)
+    Cp = nat.ptr_array([C.data_ptr() for C in Cs])
+    nocc = nat.i64_array([C.shape[1] for C in Cs])
+    ldc = nat.i64_array([C.stride(0) for C in Cs])
+    Jp = nat.ptr_array([x.data_ptr() for x in J])
+    Kp = nat.ptr_array([x.data_ptr() for x in K])
+    flags = nat.F2JK_WANT_J | nat.F2JK_WANT_K
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    if noalloc:
+        rc = lib.f2jk_build_jk_noalloc(h, len(Cs), Cp, nocc, ldc, Jp, Kp, nat.PREC[prec], flags, st)
+    else:
+        sz = ctypes.c_size_t()
+        nat.jk_check(lib.f2jk_workspace_size(h, len(Cs), nocc, nat.PREC[prec], flags, ctypes.byref(sz)), "ws")
+        ws = torch.empty(sz.value, dtype=torch.uint8, device=dev)
+        rc = lib.f2jk_build_jk(h, len(Cs), Cp, nocc, ldc, Jp, Kp, nat.PREC[prec], flags,
+                               ctypes.c_void_p(ws.data_ptr()), sz.value, st)
+    nat.jk_check(rc, "build")
+    torch.cuda.synchronize(dev)
+    return J, K
+
+
+def _case(dev, Q, N, o, seed, uhf=False):
+    B = synthetic_B(Q, N, seed=seed, device=dev)
+    Cs = [orthonormal_C(N, o, seed=seed, device=dev)]
+    if uhf:
+        Cs.append(orthonormal_C(N, max(o - 3, 1), seed=seed + 1, device=dev))
+    return B, Cs
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_noalloc_matches_build_jk_on_two_calls(dev, prec):
+    lib = nat.jk_lib()
+    # first call: (H2O)_8 shapes with two densities, whose default X chunk and
+    # split count do not fit the scratch array (fitted launch shapes); second
+    # call: one water molecule, smaller, in the same array
+    for (Q, N, o, seed, uhf) in ((1112, 192, 40, 8, True), (139, 24, 5, 1, False)):
+        print(f"[no-alloc] {prec} Q={Q} N={N} o={o} ndens={2 if uhf else 1}", flush=True)
+        B, Cs = _case(dev, Q, N, o, seed, uhf)
+        B32 = B.to(torch.float32) if prec != "fp64" else None
+        h = _ctx(dev, B, B32)
+        try:
+            Jn, Kn = _build(h, Cs, prec, True, dev)
+            Jw, Kw = _build(h, Cs, prec, False, dev)
+            for a, b in zip(Jn + Kn, Jw + Kw):
+                assert torch.isfinite(a).all()
+                if prec == "fp64":
+                    assert float((a - b).abs().max()) <= 1e-10
+                else:
+                    assert float((a - b).abs().max()) <= 2e-5 * float(b.abs().max())
+            if N == 24:  # the context's launch shapes fit: the same kernels, the same order
+                for a, b in zip(Jn + Kn, Jw + Kw):
+                    assert torch.equal(a, b)
+        finally:
+            lib.f2jk_ctx_destroy(h)
+        del B, B32, Cs
+
+
+def test_noalloc_refuses_a_build_beyond_its_scratch(dev):
+    lib = nat.jk_lib()
+    # nbf 1536 ((H2O)_64 size) needs more than the array holds even at one P
+    # per chunk and one split: refused before any launch
+    Q, N, o = 4, 1536, 320
+    B = torch.zeros(Q, N, N, dtype=torch.float64, device=dev)
+    C = torch.zeros(N, o, dtype=torch.float64, device=dev)
+    h = _ctx(dev, B)
+    try:
+        J, K = _outs(1, N, dev), _outs(1, N, dev)
+        rc = lib.f2jk_build_jk_noalloc(h, 1, nat.ptr_array([C.data_ptr()]), nat.i64_array([o]),
+                                       nat.i64_array([o]), nat.ptr_array([J[0].data_ptr()]),
+                                       nat.ptr_array([K[0].data_ptr()]), nat.PREC["fp64"],
+                                       nat.F2JK_WANT_J | nat.F2JK_WANT_K, None)
+        assert rc == ENOMEM
+        assert b"no-alloc scratch" in lib.f2jk_last_error()
+    finally:
+        lib.f2jk_ctx_destroy(h)
