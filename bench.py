@@ -91,6 +91,8 @@ def parse():
                     help="run rocprofv3 FETCH_SIZE / WRITE_SIZE passes over --profile-steps and a plain "
                          "copy as child processes and write the per-launch HBM traffic table")
     ap.add_argument("--traffic-out", default=None, help="output of --measure-traffic (default: the package table)")
+    ap.add_argument("--set", action="append", default=[], metavar="KNOB=VALUE",
+                    help="launch-shape knob of the headline builds (f2jk_set_tuning), over the tuned table")
     ap.add_argument("--scf-leg", type=int, default=1,
                     help="1: also run the real (H2O)_n RHF in FP64 and in the precision form after the "
                          "microbenchmark and report the final-energy error (0: skip)")
@@ -599,6 +601,9 @@ def main():
     C = orthonormal_C(N, o, seed=2024, device=dev)
     fb = FockBuilder(B=B, device=dev)
     fb.B_fp32()
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        fb.set_tuning(k, int(v))
     torch.cuda.synchronize()
     print(f"[bench] (H2O)_{args.waters}: N={N} Q={Q} o={o} inputs resident", file=sys.stderr, flush=True)
 
@@ -640,7 +645,11 @@ def main():
         cpu = cpu_baseline(B, C, min(args.cpu_sample_aux, Q))
         print(f"[bench] cpu baseline: {cpu['value']:.4f} builds/s on {cpu['cores']} cores",
               file=sys.stderr, flush=True)
-    kx_prefix = "f2jk::kx_bf16x3_kernel" if args.mixed == "bf16x3" else "f2jk::kx_kernel<float"
+    # the BF16x3 half-transform runs in 256-row tiles (kx_bf16x3_tall_kernel)
+    # unless the launch shape says 128 (the library's rule: 256 for nbf >= 768)
+    rows = fb.knobs.get("kx_rows", 0) or (256 if N >= 768 and N % 4 == 0 else 128)
+    kx_name = "kx_bf16x3_tall_kernel" if rows == 256 else "kx_bf16x3_kernel"
+    kx_prefix = f"f2jk::{kx_name}" if args.mixed == "bf16x3" else "f2jk::kx_kernel<float"
     traffic, traffic_note = measured_traffic(args.waters, args.mixed, kx_prefix)
 
     # the synthetic B leaves HBM before the real-integral legs
@@ -704,7 +713,7 @@ def main():
             "flops_algorithmic": kf, "flops_full_gemm": 4.0 * Q * N * N * o,
         },
         "roofline": {
-            "kernel": "kx_bf16x3_kernel (K half-transform X = B C)" if args.mixed == "bf16x3"
+            "kernel": f"{kx_name} (K half-transform X = B C, {rows}-row tiles)" if args.mixed == "bf16x3"
                       else "kx_kernel (K half-transform X = B C)",
             "bound": "mfma",
             "achieved": achieved,

@@ -12,7 +12,13 @@ the launch shape:
   * ``bf16_nbw_max`` (BF16x3): the widest half-transform tile, 32*NBW
     columns.  Narrower tiles use fewer registers and LDS but re-read the B
     rows once per column tile;
-  * ``kx_waves`` (BF16x3): 4 or 8 waves per half-transform workgroup;
+  * ``kx_waves`` (BF16x3): 4 or 8 waves per half-transform workgroup of
+    the 128-row kernel;
+  * ``ks_dma`` is not tuned: the contraction with X tiles loaded straight
+    into LDS is bitwise the register-staged one and is used with the
+    256-wide tiles, where it measured faster;
+  * ``kx_rows`` (BF16x3, single-job): 128 or 256, the rows of a
+    half-transform tile (256: the tall kernel, 2 x 4 waves, even widths);
   * ``ks_tile`` (BF16x3, single-job): 128 or 256, the lower-tile edge of the
     contraction.
 
@@ -44,10 +50,11 @@ CANDIDATES = {
     "bf16_nbw_max": [4, 5, 6, 8, 10],
     "kx_waves": [4, 8],
     "ks_tile": [128, 256],
+    "kx_rows": [128, 256],
 }
 # the BF16x3 grid is larger (tile cap, waves): fewer chunk sizes there
 XCHUNK_BF16 = [1 << 31, 1 << 32]
-KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max", "kx_waves", "ks_tile")
+KEYS = ("target_blocks", "xchunk_bytes", "bf16_nbw_max", "kx_waves", "ks_tile", "kx_rows")
 
 
 def _key(nbf, naux, precision, grouped=False):
@@ -68,10 +75,13 @@ def _candidates(precision, grouped, nbf=None):
         if not grouped:  # grouped builds use 4-wave half-transform and 128-wide contraction tiles
             names.append("kx_waves")
             if nbf is None or nbf >= KS_TILE_MIN_NBF:
-                names.append("ks_tile")
+                names += ["ks_tile", "kx_rows"]
     pools = {**CANDIDATES, "xchunk_bytes": XCHUNK_BF16 if precision == "bf16x3" else CANDIDATES["xchunk_bytes"]}
     for vals in itertools.product(*[pools[n] for n in names]):
-        yield dict(zip(names, vals))
+        cand = dict(zip(names, vals))
+        if cand.get("kx_rows") == 256 and cand.get("kx_waves") == 4:
+            continue  # the tall kernel has one wave count: the same launch as kx_waves 8
+        yield cand
 
 
 def _set(fb, cand):

@@ -64,6 +64,8 @@ struct f2jk_ctx {
     bool overlap_jk = false;
     int kx_waves = 8;               // waves per BF16x3 half-transform workgroup (4 or 8)
     int ks_tile = 0;                // BF16x3 contraction tile edge: 128, 256, 0 = by nbf
+    int kx_rows = 0;                // BF16x3 half-transform tile rows: 128, 256, 0 = by nbf
+    int ks_dma = -1;                // BF16x3 contraction tiles by direct-to-LDS loads: 0, 1, -1 = with 256 tiles
     hipStream_t side = nullptr;     // default priority, created with the context
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int target_blocks = kTargetBlocks;
@@ -95,6 +97,8 @@ struct Plan {
     // K
     int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0, kx_waves = 4;
     int ks_tile = 128;  // edge of the contraction's lower tiles (256: BF16x3 8-wave tiles)
+    int kx_rows = 128;  // rows of a BF16x3 half-transform tile (256: the tall kernel)
+    int ctrows = 0;     // rows of the padded bf16 coefficient planes (>= ncolt * tile columns)
     // workspace offsets
     int KP = 0;                // nbf rounded up to the bf16 k step
     size_t off_cp32 = 0, off_cth = 0, off_ctl = 0;
@@ -181,10 +185,13 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         off += align_up((size_t)N * p.OPAD * 4);  // fp32 copy (FP32 K)
         p.KP = (int)(cdiv(N, kBK16) * kBK16);
         if (precision == F2JK_PREC_BF16X3) {
-            p.off_cth = off;                       // transposed hi/lo bf16 copies (BF16X3 K)
-            off += align_up((size_t)p.OPAD * p.KP * 2);
+            // transposed hi/lo bf16 copies (BF16X3 K), with zero rows up to a
+            // whole number of half-transform column tiles (at most 320 wide)
+            p.ctrows = p.OPAD + 320;
+            p.off_cth = off;
+            off += align_up((size_t)p.ctrows * p.KP * 2);
             p.off_ctl = off;
-            off += align_up((size_t)p.OPAD * p.KP * 2);
+            off += align_up((size_t)p.ctrows * p.KP * 2);
         }
     }
     if (p.want_j) {
@@ -228,14 +235,27 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         const int64_t wg_target = p.ks_tile == 256 ? target_blocks / 4 : target_blocks / 2;
         p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(wg_target, p.nlower)));
         if (precision == F2JK_PREC_BF16X3) {
+            // single-job builds of large nbf run the tall (256-row) half-transform
+            p.kx_rows = 128;
+            if (!grouped) p.kx_rows = ctx->kx_rows ? ctx->kx_rows : (N >= 768 ? 256 : 128);
+            if (N % 4 || N < 32) p.kx_rows = 128;  // the tall kernel loads A rows 16 bytes at a time
             // the tile width (in 32-column units) within the cap that computes
-            // the fewest column blocks; ties go to the wider tile
+            // the fewest column blocks; ties go to the wider tile.  The tall
+            // kernel's 2 x 4 waves take even widths only.
             const int need = (int)cdiv(p.OPAD, 32);
             int best = 0, cost = 0;
             for (int w : {10, 8, 6, 5, 4, 2}) {
-                if (w > ctx->bf16_nbw_max) continue;
+                if (w > ctx->bf16_nbw_max || (p.kx_rows == 256 && w % 2)) continue;
                 const int c = (int)cdiv(need, w) * w;
                 if (best == 0 || c < cost) best = w, cost = c;
+            }
+            if (best == 0) {  // a cap of 5 leaves no even width: the 128-row kernel
+                p.kx_rows = 128;
+                for (int w : {10, 8, 6, 5, 4, 2}) {
+                    if (w > ctx->bf16_nbw_max) continue;
+                    const int c = (int)cdiv(need, w) * w;
+                    if (best == 0 || c < cost) best = w, cost = c;
+                }
             }
             p.nbw = best;
             p.kx_waves = ctx->kx_waves;
@@ -384,6 +404,15 @@ int pack_c(f2jk_ctx* ctx, const Plan& p, const double* const* C, const int64_t* 
             F2JK_LAUNCHED("pad_ct_bf16");
         }
     }
+    if (p.want_k && p.prec == F2JK_PREC_BF16X3) {
+        // zero rows past the occupied columns: the tall half-transform loads
+        // whole column tiles without clamping
+        const int zr = p.ctrows - p.OPAD;
+        const int64_t nt = (int64_t)zr * p.KP;
+        hipLaunchKernelGGL(pad_ct_bf16_kernel, dim3((unsigned)cdiv(nt, 256)), dim3(256), 0, st, C[0], 1, p.N, 0,
+                           zr, p.OPAD, p.KP, (u16*)(ws + p.off_cth), (u16*)(ws + p.off_ctl));
+        F2JK_LAUNCHED("pad_ct_bf16 zero rows");
+    }
     return F2JK_OK;
 }
 
@@ -478,6 +507,14 @@ void launch_kx_bf16(const Plan& p, unsigned grid, const float* A, int64_t R, con
     launch_kx_bf16_w<NBW, 4>(p, grid, A, R, th, tl, xh, xl, st);
 }
 
+// the tall (256-row, 2 x 4 waves) half-transform; even widths only
+template <int NBW>
+void launch_kx_bf16_tall(const Plan& p, unsigned grid, const float* A, int64_t R, const u16* th,
+                         const u16* tl, u16* xh, u16* xl, hipStream_t st) {
+    hipLaunchKernelGGL((kx_bf16x3_tall_kernel<NBW, 2, 4>), dim3(grid), dim3(512), 0, st, A, R, p.N, p.KP, th, tl,
+                       p.OPAD, xh, xl, p.ncolt);
+}
+
 int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStream_t st) {
     const float* B = (const float*)ctx->B[F2JK_F32];
     const u16* th = (const u16*)(ws + p.off_cth);
@@ -491,9 +528,19 @@ int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipSt
         const int P0 = c * p.Pc;
         const int pc = std::min(p.Pc, p.Q - P0);
         const int64_t R = (int64_t)pc * p.N;
-        const unsigned grid = (unsigned)(cdiv(R, kTile) * p.ncolt);
         const float* A = B + (int64_t)P0 * NN;
         TIM(F2JK_PH_KX, true);
+        if (p.kx_rows == 256) {
+            const unsigned grid = (unsigned)(cdiv(R, kTallRows) * p.ncolt);
+            switch (p.nbw) {
+                case 2: launch_kx_bf16_tall<2>(p, grid, A, R, th, tl, xh, xl, st); break;
+                case 4: launch_kx_bf16_tall<4>(p, grid, A, R, th, tl, xh, xl, st); break;
+                case 6: launch_kx_bf16_tall<6>(p, grid, A, R, th, tl, xh, xl, st); break;
+                case 8: launch_kx_bf16_tall<8>(p, grid, A, R, th, tl, xh, xl, st); break;
+                default: launch_kx_bf16_tall<10>(p, grid, A, R, th, tl, xh, xl, st); break;
+            }
+        } else {
+        const unsigned grid = (unsigned)(cdiv(R, kTile) * p.ncolt);
         switch (p.nbw) {
             case 2: launch_kx_bf16<2>(p, grid, A, R, th, tl, xh, xl, st); break;
             case 4: launch_kx_bf16<4>(p, grid, A, R, th, tl, xh, xl, st); break;
@@ -502,12 +549,22 @@ int run_k_bf16x3(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipSt
             case 8: launch_kx_bf16<8>(p, grid, A, R, th, tl, xh, xl, st); break;
             default: launch_kx_bf16<10>(p, grid, A, R, th, tl, xh, xl, st); break;
         }
+        }
         F2JK_LAUNCHED("kx_bf16x3");
         TIM(F2JK_PH_KX, false);
         TIM(F2JK_PH_KS, true);
         const int nsplit = std::min(p.nsplit, pc);
+        const bool ks_dma = ctx->ks_dma < 0 ? p.ks_tile == 256 : ctx->ks_dma == 1;
         for (int d = 0; d < p.nd; ++d) {
-            if (p.ks_tile == 256)
+            if (ks_dma && p.ks_tile == 256)
+                hipLaunchKernelGGL((ks_bf16x3_dma_kernel<256, 4, 2>), dim3(p.nlower * nsplit), dim3(512), 0, st,
+                                   xh, xl, p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
+                                   kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
+            else if (ks_dma)
+                hipLaunchKernelGGL((ks_bf16x3_dma_kernel<kTile, 2, 2>), dim3(p.nlower * nsplit), dim3(kGThreads), 0,
+                                   st, xh, xl, p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
+                                   kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
+            else if (p.ks_tile == 256)
                 hipLaunchKernelGGL((ks_bf16x3_kernel<256, 4, 2>), dim3(p.nlower * nsplit), dim3(512), 0, st,
                                    xh, xl, p.N, p.OPAD, p.ioff[d], p.width[d], pc, nsplit, p.nlower,
                                    kpart + (size_t)d * p.nsplit * NN, c > 0 ? 1 : 0);
@@ -829,6 +886,12 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
         ctx->xchunk_bytes = (size_t)value;
     } else if (!strcmp(key, "j_nontemporal")) {
         ctx->j_nontemporal = value != 0;
+    } else if (!strcmp(key, "ks_dma")) {
+        if (value < -1 || value > 1) return fail(F2JK_EINVAL, "ks_dma must be -1, 0 or 1");
+        ctx->ks_dma = (int)value;
+    } else if (!strcmp(key, "kx_rows")) {
+        if (value != 0 && value != 128 && value != 256) return fail(F2JK_EINVAL, "kx_rows must be 0, 128 or 256");
+        ctx->kx_rows = (int)value;
     } else if (!strcmp(key, "bf16_nbw_max")) {
         if (value != 2 && value != 4 && value != 5 && value != 6 && value != 8 && value != 10)
             return fail(F2JK_EINVAL, "bf16_nbw_max must be 2, 4, 5, 6, 8 or 10");

@@ -940,6 +940,215 @@ __global__ __launch_bounds__(64 * WAVES, (kx_bf16_waves_per_eu<NBW, WAVES>())) v
 }
 
 
+// ---------------------------------------------------------------------------
+// KX (bf16x3), tall tiles: the same product as kx_bf16x3_kernel with 256-row
+// tiles, so that each staged coefficient tile feeds twice the rows (the
+// coefficient tile, 2.5x the bytes of a 128-row A tile at 320 columns, is
+// what the 128-row kernel spends most of its staging on).  Per k-step:
+//   * A (FP32 rows of B) is loaded into registers one step ahead, split into
+//     bf16 hi/lo and stored to the next LDS stage at the start of the step;
+//   * the pre-split coefficient tile (hi and lo planes) goes from global
+//     memory straight into the next LDS stage (global_load_lds_dwordx4, no
+//     registers): a wave's instruction fills 1 KB = 16 swizzled 64-byte rows,
+//     lane l writing slot l, so lane l fetches the logical chunk that the
+//     swizzle puts in slot l (the XOR swizzle is its own inverse);
+//   * the MFMAs of the current stage run under both.
+// The direct-to-LDS loads are issued from inline asm: the compiler does not
+// track them, so it never waits for them before unrelated LDS reads (its own
+// tracking waits for every pending one before any LDS read).  The kernel
+// waits for them itself, with vmcnt(0) at the end of each step (the register
+// loads of A are waited for too: a direct-to-LDS load need not complete in
+// order with register loads, so no partial count is relied on); the barrier
+// then publishes them.  Both kinds of load are issued at the start of the
+// step, a full step ahead of their use.
+// WR x WC waves; each wave holds a (256/WR) x (32*NBW/WC) block of the tile.
+// ---------------------------------------------------------------------------
+constexpr int kTallRows = 256;
+
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+// 16 bytes per lane from global memory (wave-uniform base + per-lane 32-bit
+// byte offset) into LDS at m0 + 16 * lane
+__device__ __forceinline__ void lds_load16(const void* base, uint32_t off, uint32_t lds_base) {
+    asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(off), "s"(base), "s"(lds_base)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+// Every global load (direct-to-LDS ones included) and LDS access of this wave
+// has completed, then the workgroup barrier.  One asm statement with a memory
+// clobber: the compiler moves no memory access across it.
+__device__ __forceinline__ void drain_and_barrier() {
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+// every global load of this wave has completed
+__device__ __forceinline__ void drain_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+template <int NBW, int WR, int WC>
+__device__ __forceinline__ void kx_bf16x3_tall_body(
+    const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
+    const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt, int b,
+    int nblocks) {
+    constexpr int TM = kTallRows;
+    constexpr int BN = 32 * NBW;
+    constexpr int NW = WR * WC;
+    constexpr int T = 64 * NW;
+    constexpr int IB = TM / WR / 16;     // 16-row blocks per wave
+    constexpr int NB = BN / WC / 16;     // 16-column blocks per wave
+    static_assert(TM % (16 * WR) == 0 && BN % (16 * WC) == 0 && NB >= 1, "tile must split over the waves");
+    // A: TM x 32 FP32 = TM*8 16-byte chunks, VA = 4 elements per chunk
+    constexpr int A_LD = TM * 8 / T;
+    static_assert(A_LD * T == TM * 8 && A_LD <= 15, "A tile must split evenly");
+    // coefficient tile: 2 planes x BN rows x 4 chunks, in wave instructions of 16 rows
+    constexpr int B_INS = 2 * BN / 16;
+    constexpr int B_LD = B_INS / NW;
+    static_assert(B_LD * NW == B_INS, "coefficient tile must split over the waves");
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[2][TM][kBK16];
+    __shared__ __attribute__((aligned(16))) __bf16 Al[2][TM][kBK16];
+    __shared__ __attribute__((aligned(16))) __bf16 Bt[2][2][BN][kBK16];  // [stage][hi, lo][row][k]
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = xcd_remap(b, nblocks);
+    const int64_t r0 = (int64_t)(t / ncolt) * TM;
+    const int c0 = (t % ncolt) * BN;
+    const int wr = wid / WC, wc = wid % WC;
+    const int wcol0 = c0 + wc * 16 * NB;
+    int nbv = (OPAD - wcol0) / 16;
+    nbv = nbv < 0 ? 0 : (nbv > NB ? NB : nbv);
+
+    Vec<float, 4> ra[A_LD];
+    // Rows past R are clamped onto the last row (their outputs are never
+    // stored); k past K is clamped onto the row's last 4 elements, finite
+    // values that meet the zero padding of Ct.  The tile's base is
+    // wave-uniform; each load adds a loop-invariant 32-bit row offset.
+    const float* Atile = A + r0 * K;
+    uint32_t aoff[A_LD];
+    static_for<0, A_LD>([&](auto S) {
+        constexpr int s = decltype(S)::value;
+        const int row = (tid + T * s) >> 3;
+        aoff[s] = (uint32_t)((min(r0 + row, R - 1) - r0) * K);
+    });
+    auto load_a = [&](int k0) {
+        static_for<0, A_LD>([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            const int kc = ((tid + T * s) & 7) * 4;
+            ra[s] = load_vec<float, 4>(Atile + aoff[s] + min(k0 + kc, K - 4));
+        });
+    };
+    auto store_a = [&](int stg) {
+        static_for<0, A_LD>([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            const int idx = tid + T * s;
+            const int row = idx >> 3, kc = (idx & 7) * 4;
+            const int e = swz16(row, kc >> 3) + (kc & 7);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) split_bf16(ra[s].v[v], Ah[stg][row][e + v], Al[stg][row][e + v]);
+        });
+    };
+    // This wave's direct-to-LDS loads of the coefficient tile of step k0.
+    // Ct has ncolt * BN rows (zero past OPAD), so no row is clamped: a load's
+    // address is a wave-uniform base (plane, 16-row group, k0) plus the
+    // lane's offset, the same for every load (the swizzle of row 16g + lrow
+    // depends on lrow only).
+    const int lrow = lane >> 2, lpos = lane & 3;
+    const uint32_t boff = (uint32_t)(lrow * KP + (lpos ^ ((lrow >> 2) & 2)) * 8) * 2;
+    auto load_b = [&](int k0, int stg) {
+        static_for<0, B_LD>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const int g = wid + NW * q;                // wave instruction index (wave-uniform)
+            const int plane = g / (BN / 16), grp = g % (BN / 16);
+            const u16* base = (plane ? Ctl : Cth) + (int64_t)(c0 + grp * 16) * KP + k0;
+            lds_load16(base, boff, lds_addr(&Bt[stg][plane][grp * 16][0]));
+        });
+    };
+
+    fx4 acc[IB][NB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = fx4{0, 0, 0, 0};
+
+    const int nk = KP / kBK16;
+    load_a(0);
+    store_a(0);
+    load_b(0, 0);
+    load_a((nk > 1 ? 1 : 0) * kBK16);
+    drain_and_barrier();
+
+    const int fr = lane & 15, fc = lane >> 4;
+    const int ra0 = wr * (TM / WR), rb0 = wc * 16 * NB;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1, nxt = cur ^ 1;
+        const int k1 = min(kt + 1, nk - 1) * kBK16;  // clamped: the last steps repeat harmless loads
+        const int k2 = min(kt + 2, nk - 1) * kBK16;
+        bf16x8 bh[NB], bl[NB], ah[2], al[2];
+        // row block 0 streams the column fragments in; later row blocks reuse them
+        {
+            const int r = ra0 + fr;
+            al[0] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
+            ah[0] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
+        }
+        static_for<0, NB>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const int r = rb0 + j * 16 + fr;
+            bh[j] = *reinterpret_cast<const bf16x8*>(&Bt[cur][0][r][swz16(r, fc)]);
+            bl[j] = *reinterpret_cast<const bf16x8*>(&Bt[cur][1][r][swz16(r, fc)]);
+        });
+        static_for<0, IB>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if constexpr (i + 1 < IB) {
+                const int r = ra0 + (i + 1) * 16 + fr;
+                al[(i + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Al[cur][r][swz16(r, fc)]);
+                ah[(i + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Ah[cur][r][swz16(r, fc)]);
+            }
+            // three sweeps over the column blocks: consecutive MFMAs never share an accumulator
+            static_for<0, NB>([&](auto J) { acc[i][J] = mfma_bf16(al[i & 1], bh[J], acc[i][J]); });
+            static_for<0, NB>([&](auto J) { acc[i][J] = mfma_bf16(ah[i & 1], bl[J], acc[i][J]); });
+            static_for<0, NB>([&](auto J) { acc[i][J] = mfma_bf16(ah[i & 1], bh[J], acc[i][J]); });
+            // staging of the next steps, under the MFMAs of row block 0: the
+            // split of A(kt+1) (loaded in the previous step), the register
+            // loads of A(kt+2), the LDS loads of Ct(kt+1)
+            if constexpr (i == 0) {
+                store_a(nxt);
+                load_a(k2);
+                load_b(k1, nxt);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        // every load and LDS store of this wave has landed, then publish
+        drain_and_barrier();
+    }
+    // no direct-to-LDS load may outlive the workgroup
+    drain_vm();
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j >= nbv) continue;
+            const int col = wcol0 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + ra0 + i * 16 + 4 * (lane >> 4) + r;
+                if (row < R) {
+                    __bf16 h, l;
+                    split_bf16(acc[i][j][r], h, l);
+                    Xh[row * OPAD + col] = __builtin_bit_cast(u16, h);
+                    Xl[row * OPAD + col] = __builtin_bit_cast(u16, l);
+                }
+            }
+        }
+}
+
+template <int NBW, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC, (WR * WC == 8 ? 2 : 1)) void kx_bf16x3_tall_kernel(
+    const float* __restrict__ A, int64_t R, int K, int KP, const u16* __restrict__ Cth,
+    const u16* __restrict__ Ctl, int OPAD, u16* __restrict__ Xh, u16* __restrict__ Xl, int ncolt) {
+    kx_bf16x3_tall_body<NBW, WR, WC>(A, R, K, KP, Cth, Ctl, OPAD, Xh, Xl, ncolt, blockIdx.x, gridDim.x);
+}
+
 // KS (bf16x3): Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]
 // with X given as hi/lo bf16 pairs; width % 32 == 0.  FP32 accumulation over
 // the P range of a split (the MFMA rounds once per 32 products), FP64 across
@@ -1123,6 +1332,136 @@ __global__ __launch_bounds__(64 * WR * WC, (TS == 128 ? 2 : 1)) void ks_bf16x3_k
                                       blockIdx.x, gridDim.x);
 }
 
+
+// KS (bf16x3) with the tiles of X brought into LDS by direct-to-LDS loads:
+// the same product, tiles, split ranges and MFMA sequence as
+// ks_bf16x3_kernel (so the same Kpart, bitwise), without staging registers
+// or LDS stores.  Each step issues the loads of the next step into the other
+// stage right after the barrier and waits for them (vmcnt 0: no other global
+// loads are in flight in the loop) at the end of the step.  A wave
+// instruction fills 16 swizzled 64-byte rows of one plane (A hi, A lo, B hi,
+// B lo); rows past N are clamped onto row N-1 (per lane, loop-invariant).
+template <int TS, int WR, int WC>
+__device__ __forceinline__ void ks_bf16x3_dma_body(
+    const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
+    int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate, int b, int nblocks) {
+    constexpr int NW = WR * WC;
+    constexpr int IB = TS / WR / 16;
+    constexpr int JB = TS / WC / 16;
+    constexpr int INS = 4 * TS / 16;   // wave instructions per step: 4 planes x TS/16 row groups
+    constexpr int LD = INS / NW;
+    static_assert(LD * NW == INS && JB >= 2, "tile must split over the waves");
+    __shared__ __attribute__((aligned(16))) __bf16 Xt[2][4][TS][kBK16];  // [stage][Ah, Al, Bh, Bl][row][k]
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lt = xcd_remap(b, nblocks);
+    const int s = lt / nlower;
+    const int tl = lt % nlower;
+    int mt = 0;
+    while ((mt + 1) * (mt + 2) / 2 <= tl) ++mt;
+    const int nt = tl - mt * (mt + 1) / 2;
+    const int m0 = mt * TS, n0 = nt * TS;
+    const int wr = wid / WC, wc = wid % WC;
+    const int Pa = (int)(((int64_t)Pc * s) / nsplit);
+    const int Pb = (int)(((int64_t)Pc * (s + 1)) / nsplit);
+
+    // per load q: wave-uniform row of its 16-row group (clamped) and the
+    // lane's 32-bit byte offset from it (clamped row, swizzled chunk)
+    const int lrow = lane >> 2, lpos = lane & 3;
+    const int lchunk = lpos ^ ((lrow >> 2) & 2);
+    int qrow[LD];
+    uint32_t qoff[LD];
+    static_for<0, LD>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        const int g = wid + NW * q;
+        const int plane = g / (TS / 16), grp = g % (TS / 16);
+        const int r0 = (plane < 2 ? m0 : n0) + grp * 16;
+        const int rb = min(r0, N - 1);
+        qrow[q] = rb;
+        qoff[q] = (uint32_t)(((min(r0 + lrow, N - 1) - rb) * OPAD + lchunk * 8) * 2);
+    });
+    auto load = [&](int P, int i0, int stg) {
+        static_for<0, LD>([&](auto Q) {
+            constexpr int q = decltype(Q)::value;
+            const int g = wid + NW * q;
+            const int plane = g / (TS / 16), grp = g % (TS / 16);
+            const u16* base = ((plane & 1) ? Xl : Xh) + ((int64_t)P * N + qrow[q]) * OPAD + ioff + i0;
+            lds_load16(base, qoff[q], lds_addr(&Xt[stg][plane][grp * 16][0]));
+        });
+    };
+
+    fx4 acc[IB][JB];
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = fx4{0, 0, 0, 0};
+
+    const int nkp = width / kBK16;
+    const int nsteps = (Pb - Pa) * nkp;
+    if (nsteps > 0) load(Pa, 0, 0);
+    drain_and_barrier();
+    const int fr = lane & 15, fc = lane >> 4;
+    const int ra0 = wr * (TS / WR), rb0w = wc * (TS / WC);
+    int st = 0;
+    for (int P = Pa; P < Pb; ++P) {
+    for (int kt = 0; kt < nkp; ++kt, ++st) {
+        const int cur = st & 1, nxt = cur ^ 1;
+        const int sp = min(st + 1, nsteps - 1);  // clamped: the last loads are harmless repeats
+        load(Pa + sp / nkp, (sp % nkp) * kBK16, nxt);
+        bf16x8 ah[IB], al[IB], bh[2], bl[2];
+        const int rb0 = rb0w + fr;
+        bh[0] = *reinterpret_cast<const bf16x8*>(&Xt[cur][2][rb0][swz16(rb0, fc)]);
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+            const int r = ra0 + i * 16 + fr;
+            al[i] = *reinterpret_cast<const bf16x8*>(&Xt[cur][1][r][swz16(r, fc)]);
+        }
+        bl[0] = *reinterpret_cast<const bf16x8*>(&Xt[cur][3][rb0][swz16(rb0, fc)]);
+#pragma unroll
+        for (int i = 0; i < IB; ++i) {
+            const int r = ra0 + i * 16 + fr;
+            ah[i] = *reinterpret_cast<const bf16x8*>(&Xt[cur][0][r][swz16(r, fc)]);
+        }
+        static_for<0, JB>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            if constexpr (j + 1 < JB) {
+                const int r = rb0w + (j + 1) * 16 + fr;
+                bh[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Xt[cur][2][r][swz16(r, fc)]);
+                bl[(j + 1) & 1] = *reinterpret_cast<const bf16x8*>(&Xt[cur][3][r][swz16(r, fc)]);
+            }
+            static_for<0, IB>([&](auto I) { acc[I][j] = mfma_bf16(al[I], bh[j & 1], acc[I][j]); });
+            static_for<0, IB>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bl[j & 1], acc[I][j]); });
+            static_for<0, IB>([&](auto I) { acc[I][j] = mfma_bf16(ah[I], bh[j & 1], acc[I][j]); });
+            __builtin_amdgcn_sched_barrier(0);
+        });
+        drain_and_barrier();
+    }
+    }
+    double* Ks = Kpart + (int64_t)s * N * N;
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) {
+            const int col = n0 + rb0w + j * 16 + (lane & 15);
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + ra0 + i * 16 + 4 * (lane >> 4) + r;
+                if (row >= N) continue;
+                double* dst = Ks + (int64_t)row * N + col;
+                const double v = (double)acc[i][j][r];
+                *dst = accumulate ? (*dst + v) : v;
+            }
+        }
+}
+template <int TS, int WR, int WC>
+__global__ __launch_bounds__(64 * WR * WC, (TS == 128 ? 2 : 1)) void ks_bf16x3_dma_kernel(
+    const u16* __restrict__ Xh, const u16* __restrict__ Xl, int N, int OPAD, int ioff, int width,
+    int Pc, int nsplit, int nlower, double* __restrict__ Kpart, int accumulate) {
+    ks_bf16x3_dma_body<TS, WR, WC>(Xh, Xl, N, OPAD, ioff, width, Pc, nsplit, nlower, Kpart, accumulate,
+                                   blockIdx.x, gridDim.x);
+}
 
 // out[r][c] = X[r][c] for r < R, c < ncol: a half-transformed chunk (row
 // stride OPAD) copied to a caller buffer with row stride ldx.

@@ -60,6 +60,7 @@ class FockBuilder:
         self._ws = None
         self._manual = set()  # launch-shape knobs set by hand
         self._tuned = None    # precision whose tuned launch shape is applied
+        self.knobs = {}       # launch-shape knobs set on the context (by hand or from the table)
         self.timing = False
         self.last_timings = None
         self.last_launches = None
@@ -135,6 +136,7 @@ class FockBuilder:
 
     def _tune(self, key, value):
         nat.jk_check(nat.jk_lib().f2jk_set_tuning(self.ctx, key.encode(), int(value)), "f2jk_set_tuning")
+        self.knobs[key] = int(value)
 
     def _apply_table(self, prec):
         """Launch shape of this size class and precision from the autotuned

@@ -99,7 +99,13 @@ int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
  * a BF16x3 half-transform workgroup (8: two per SIMD sharing the tile; tiles
  * of an even number of 32-column units); "ks_tile" (0, 128, 256; default 0 =
  * 256 when nbf >= 768) is the lower-tile edge of the single-job BF16x3
- * contraction (256: one 8-wave workgroup per CU; grouped builds use 128).
+ * contraction (256: one 8-wave workgroup per CU; grouped builds use 128);
+ * "kx_rows" (0, 128, 256; default 0 = 256 when nbf >= 768 and nbf % 4 == 0)
+ * is the row count of a single-job BF16x3 half-transform tile (256: 8 waves,
+ * even widths, coefficient tiles loaded straight into LDS); "ks_dma" (-1, 0,
+ * 1; default -1 = with 256-wide tiles) loads the BF16x3 contraction's X
+ * tiles straight into LDS instead of staging them through registers (the same
+ * products, bitwise).
  * Set by the autotuner. */
 int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value);
 

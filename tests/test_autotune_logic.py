@@ -54,7 +54,11 @@ def test_candidate_sets():
     grouped_bf = list(autotune._candidates("bf16x3", grouped=True))
     assert len(grouped_bf) == 35 and all("xchunk_bytes" not in c and "kx_waves" not in c for c in grouped_bf)
     single_bf = list(autotune._candidates("bf16x3", grouped=False))
-    assert len(single_bf) == 280 and {c["kx_waves"] for c in single_bf} == {4, 8}
+    # 7 block targets x 2 chunk sizes x 5 widths x 2 contraction tiles x
+    # (4 or 8 waves of the 128-row half-transform, or the 256-row one)
+    assert len(single_bf) == 420 and {c["kx_waves"] for c in single_bf} == {4, 8}
     assert {c["ks_tile"] for c in single_bf} == {128, 256}
+    assert {c["kx_rows"] for c in single_bf} == {128, 256}
+    assert not any(c["kx_rows"] == 256 and c["kx_waves"] == 4 for c in single_bf)
     N, Q, o = autotune.alkane_shapes([20])[0]
     assert (N, Q, o) == (1188, 2840, [81, 80])

@@ -145,6 +145,70 @@ def test_bf16x3_precision_form(dev, Q, N, noccs, ks_tile):
         assert torch.equal(K[k], K[k].T)
 
 
+@pytest.mark.parametrize(
+    "Q,N,noccs,xchunk",
+    [
+        (2224, 384, [80], 0),            # (H2O)_16
+        (300, 132, [17, 13], 0),         # two densities, a partial last 256-row tile
+        (500, 256, [330], 0),            # two column tiles of 192 (zero coefficient rows past 352)
+        (1112, 192, [40], 64 << 20),     # several X chunks
+    ],
+)
+def test_bf16x3_tall_half_transform(dev, Q, N, noccs, xchunk):
+    # The 256-row half-transform (default for nbf >= 768, forced here) runs
+    # the same MFMA sequence per output element as the 128-row kernel (k
+    # steps in order, the three products in the same order), so K is bitwise
+    # the same; both are checked against the oracle at the split bound.
+    B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=23)
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    out = {}
+    for rows in (128, 256):
+        fb = FockBuilder(B=B, device=dev)
+        fb.set_tuning("kx_rows", rows)
+        if xchunk:
+            fb.set_tuning("xchunk_bytes", xchunk)
+        J, K = fb.build_JK(Cs, precision="bf16x3")
+        out[rows] = K
+        for k in range(len(noccs)):
+            rk = np.abs(K[k].cpu().numpy() - Kr[k]).max() / np.abs(Kr[k]).max()
+            print(f"   bf16x3 {rows}-row half-transform dens {k}: rel|dK|={rk:.2e}", flush=True)
+            assert rk < 2e-5
+        fb.release()
+    for k in range(len(noccs)):
+        assert torch.equal(out[128][k], out[256][k])
+
+
+@pytest.mark.parametrize(
+    "Q,N,noccs",
+    [
+        (2224, 384, [80]),        # (H2O)_16
+        (300, 130, [17, 13]),     # two densities, rows past nbf clamped in the last tile
+        (100, 19, [5]),           # odd nbf, one partial tile
+        (500, 256, [330]),        # two column tiles in the half-transform
+    ],
+)
+@pytest.mark.parametrize("ks_tile", [128, 256])
+def test_bf16x3_contraction_by_direct_to_lds_loads(dev, Q, N, noccs, ks_tile):
+    # The contraction with its X tiles loaded straight into LDS runs the same
+    # tiles, splits and MFMA sequence as the register-staged one: K bitwise
+    # the same, and within the split bound of the oracle.
+    B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=29)
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    out = {}
+    for dma in (0, 1):
+        fb = FockBuilder(B=B, device=dev)
+        fb.set_tuning("ks_tile", ks_tile)
+        fb.set_tuning("ks_dma", dma)
+        J, K = fb.build_JK(Cs, precision="bf16x3")
+        out[dma] = K
+        fb.release()
+    for k in range(len(noccs)):
+        rk = np.abs(out[1][k].cpu().numpy() - Kr[k]).max() / np.abs(Kr[k]).max()
+        print(f"   bf16x3 contraction tile {ks_tile} by direct-to-LDS loads, dens {k}: rel|dK|={rk:.2e}", flush=True)
+        assert rk < 2e-5
+        assert torch.equal(out[0][k], out[1][k])
+
+
 def test_error_paths(dev):
     B = synthetic_B(10, 8, seed=0, device=dev)
     fb = FockBuilder(B=B, device=dev)
