@@ -103,6 +103,8 @@ def parse():
                     help="instead of the J/K microbenchmark, run a real (H2O)_n RHF in FP64 and in the "
                          "precision form and report energies, errors and SCF latency")
     ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
+    ap.add_argument("--scf-profile", action="store_true",
+                    help="--scf: also time the SCF phases (builds, diagonalisation, rest; device syncs added)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 4 GiB device copy (x3), the reference for HBM byte counters")
     ap.add_argument("--peaks", action="store_true",
@@ -181,7 +183,7 @@ def measured_peaks(read_buf=None, read_reps=3):
     return out
 
 
-def scf_leg(n, mixed, seed=None, log=None):
+def scf_leg(n, mixed, seed=None, log=None, profile=False):
     """Real (H2O)_n cc-pVDZ/cc-pVTZ-JKFIT RHF: integrals on the host streamed into
     HBM, then the FP64 plain form and the precision form on the same B, both
     to e_tol 1e-10.  Returns energies, iterations, SCF seconds and the final
@@ -221,12 +223,17 @@ def scf_leg(n, mixed, seed=None, log=None):
         fb.timing = False
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        r = RHF(charge=0, e_tol=1e-10, precision=prec)(s).run()
+        r = RHF(charge=0, e_tol=1e-10, precision=prec)(s)
+        r.profile = profile
+        r.run()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
         stages = "".join(h[1][0] for h in r.history)
         runs[prec] = {"E": r.E, "iterations": r.iter, "scf_seconds": dt, "stages": stages,
-                      "converged": bool(r.converged), "lp_error_vs_fp64_Eh": r.lp_error_vs_fp64}
+                      "converged": bool(r.converged), "lp_error_vs_fp64_Eh": r.lp_error_vs_fp64,
+                      "dE_per_iteration": [h[2] - g[2] for g, h in zip(r.history, r.history[1:])]}
+        if profile:
+            runs[prec]["phase_seconds"] = r.phase_seconds
         log(f"{prec}: E={r.E:.12f} iters={r.iter} {dt:.2f} s stages={stages}")
     out["runs"] = runs
     out["final_energy_error_vs_fp64_Eh"] = abs(runs[mixed]["E"] - runs["fp64"]["E"])
@@ -240,7 +247,7 @@ def scf_leg(n, mixed, seed=None, log=None):
 
 def scf_mode(args):
     out = scf_leg(args.waters, args.mixed, args.seed,
-                  log=lambda m: print(f"[scf] {m}", file=sys.stderr, flush=True))
+                  log=lambda m: print(f"[scf] {m}", file=sys.stderr, flush=True), profile=args.scf_profile)
     print(json.dumps(out))
 
 

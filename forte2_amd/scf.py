@@ -162,6 +162,11 @@ class _SCF:
         self.converged = False
         self.executed = False
         self.V_sap = None  # SAP guess potential, computed from the system when not given
+        # per-phase wall times (s) when `profile` is set: J/K builds, the
+        # diagonalisation and density, the rest (DIIS, gradient, energy, host
+        # syncs); each phase is closed by a device synchronisation
+        self.profile = False
+        self.phase_seconds = None
 
     def __call__(self, system):
         self.system = system
@@ -282,14 +287,27 @@ class _SCF:
             return self.fb.build_J(args, precision=prec)
         return self.fb.build_K(args, precision=prec)
 
+    def _timed(self, phase, fn, *args):
+        if not self.profile:
+            return fn(*args)
+        torch.cuda.synchronize(self.dev)
+        t0 = time.perf_counter()
+        out = fn(*args)
+        torch.cuda.synchronize(self.dev)
+        self.phase_seconds[phase] += time.perf_counter() - t0
+        return out
+
     def run(self):
+        self.phase_seconds = {"build": 0.0, "diag": 0.0} if self.profile else None
         gen = self.iterate()
         try:
             req = next(gen)
             while True:
-                req = gen.send(self._execute(req))
+                req = gen.send(self._timed("build", self._execute, req))
         except StopIteration:
             pass
+        if self.profile:
+            self.phase_seconds["other"] = self.time - self.phase_seconds["build"] - self.phase_seconds["diag"]
         return self
 
     def iterate(self):
@@ -314,7 +332,7 @@ class _SCF:
         for _ in range(self.maxiter):
             grad = self._ao_grad(F)
             Fx = self._apply_level_shift(self._diis_update(diis, F, grad))
-            self.eps, self.C = self._diagonalize(Fx)
+            self.eps, self.C = self._timed("diag", self._diagonalize, Fx)
             self.D = self._density()
             # stage transitions of the precision form (decided on the previous dE)
             next_stage = stage
