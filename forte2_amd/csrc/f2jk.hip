@@ -57,6 +57,13 @@ struct f2jk_ctx {
     int64_t naux = 0, nbf = 0;
     int* rowoff[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // by VEC (1, 2, 4)
     int nvec[5] = {0, 0, 0, 0, 0};
+    // J over the nonzero vector elements only ("j_sparse"): per B copy (dtype)
+    // the list of lower-triangle vector elements (of that copy's J width)
+    // where B is nonzero for some P
+    bool j_sparse = false;
+    uint32_t* jlist[2] = {nullptr, nullptr};
+    int jlist_vec[2] = {0, 0};
+    int jlist_n[2] = {0, 0};
     size_t xchunk_bytes = kDefaultXChunkBytes;
     bool j_nontemporal = true;
     int bf16_nbw_max = 10;          // widest BF16x3 half-transform tile: 32 * 10 columns
@@ -93,6 +100,7 @@ struct Plan {
     int width[kMaxDens] = {}, ioff[kMaxDens] = {};
     // J
     int jvec = 1, nvec = 0, neg = 0, npc1 = 0, ppw2 = 0, npc2 = 0;
+    const uint32_t* jlist = nullptr;  // element list of the J passes (null: every element)
     size_t jb = 8;             // bytes of a B element read by J
     // K
     int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0, kx_waves = 4;
@@ -130,6 +138,61 @@ int ensure_rowoff(f2jk_ctx* ctx, int vec) {
     F2JK_HIP(hipMalloc(&ctx->rowoff[vec], sizeof(int) * (N + 1)));
     F2JK_HIP(hipMemcpy(ctx->rowoff[vec], h.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
     ctx->nvec[vec] = (int)acc;
+    return F2JK_OK;
+}
+
+// The J element list of B copy `dtype` (see f2jk_ctx::jlist): the device
+// flags the lower-triangle vector elements where B is nonzero for some P (one
+// pass over the lower triangle), the host compacts them.  Set-up work: it
+// synchronises the device.
+template <typename TB, int VEC>
+int scan_jlist(f2jk_ctx* ctx, const TB* B, unsigned char* flag) {
+    const int nvec = ctx->nvec[VEC];
+    hipLaunchKernelGGL((j_scan_kernel<TB, VEC>), dim3((unsigned)cdiv(nvec, 256)), dim3(256), 0, nullptr, B,
+                       (int)ctx->nbf, (int)ctx->naux, ctx->rowoff[VEC], nvec, flag);
+    F2JK_LAUNCHED("j_scan");
+    return F2JK_OK;
+}
+
+void free_jlist(f2jk_ctx* ctx, int dtype) {
+    if (ctx->jlist[dtype]) (void)hipFree(ctx->jlist[dtype]);
+    ctx->jlist[dtype] = nullptr;
+    ctx->jlist_vec[dtype] = ctx->jlist_n[dtype] = 0;
+}
+
+int build_jlist(f2jk_ctx* ctx, int dtype) {
+    free_jlist(ctx, dtype);
+    if (!ctx->B[dtype]) return F2JK_OK;
+    const int vec = j_vec_for(dtype, ctx->nbf);
+    int rc = ensure_rowoff(ctx, vec);
+    if (rc) return rc;
+    const int N = (int)ctx->nbf, nvec = ctx->nvec[vec];
+    F2JK_HIP(hipSetDevice(ctx->device));
+    F2JK_HIP(hipDeviceSynchronize());  // B may have been written on any stream
+    unsigned char* flag = nullptr;
+    F2JK_HIP(hipMalloc(&flag, nvec));
+    if (dtype == F2JK_F64)
+        rc = vec == 2 ? scan_jlist<double, 2>(ctx, (const double*)ctx->B[dtype], flag)
+                      : scan_jlist<double, 1>(ctx, (const double*)ctx->B[dtype], flag);
+    else
+        rc = vec == 4 ? scan_jlist<float, 4>(ctx, (const float*)ctx->B[dtype], flag)
+           : vec == 2 ? scan_jlist<float, 2>(ctx, (const float*)ctx->B[dtype], flag)
+                      : scan_jlist<float, 1>(ctx, (const float*)ctx->B[dtype], flag);
+    std::vector<unsigned char> h(nvec);
+    hipError_t e = rc ? hipSuccess : hipMemcpy(h.data(), flag, nvec, hipMemcpyDeviceToHost);
+    (void)hipFree(flag);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(F2JK_EHIP, std::string("j_scan flags: ") + hipGetErrorString(e));
+    std::vector<uint32_t> list;
+    int el = 0;
+    for (int m = 0; m < N; ++m)
+        for (int c = 0; c <= m; c += vec, ++el)
+            if (h[el]) list.push_back(((uint32_t)m << 16) | (uint32_t)c);
+    if (list.empty()) list.push_back(0u);  // an all-zero B: one element keeps the launches non-empty
+    F2JK_HIP(hipMalloc(&ctx->jlist[dtype], list.size() * sizeof(uint32_t)));
+    F2JK_HIP(hipMemcpy(ctx->jlist[dtype], list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ctx->jlist_vec[dtype] = vec;
+    ctx->jlist_n[dtype] = (int)list.size();
     return F2JK_OK;
 }
 
@@ -199,6 +262,10 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         p.jvec = jvec_override ? jvec_override : j_vec_for(jdtype, N);
         int64_t nvec = 0;
         for (int64_t m = 0; m < N; ++m) nvec += (m + 1 + p.jvec - 1) / p.jvec;
+        if (!grouped && ctx->jlist[jdtype] && ctx->jlist_vec[jdtype] == p.jvec) {
+            p.jlist = ctx->jlist[jdtype];
+            nvec = ctx->jlist_n[jdtype];
+        }
         p.nvec = (int)nvec;
         p.neg = (int)cdiv(nvec, (int64_t)kJThreads * kEPT);
         p.npc1 = (int)cdiv(Q, kJ1PerBlock);
@@ -318,9 +385,10 @@ int launch_j(f2jk_ctx* ctx, const Plan& p, const TB* B, const double* const* D, 
     double* gamma = (double*)(ws + p.off_gamma);
     double* jpart = (double*)(ws + p.off_jpart);
     const int* rowoff = ctx->rowoff[VEC];
+    const uint32_t* elist = p.jlist;
     TIM(F2JK_PH_JGAMMA, true);
     hipLaunchKernelGGL((j_gamma_kernel<TB, VEC, kEPT, ND, NT>), dim3(p.neg, p.npc1), dim3(kJThreads), 0,
-                       st, B, p.N, p.Q, dp, ldd, rowoff, p.nvec, kJ1PerBlock, gpart);
+                       st, B, p.N, p.Q, dp, ldd, rowoff, elist, p.nvec, kJ1PerBlock, gpart);
     F2JK_LAUNCHED("j_gamma");
     const int64_t ng = (int64_t)ND * p.Q;
     hipLaunchKernelGGL(j_gamma_reduce_kernel, dim3((unsigned)cdiv(ng, 256)), dim3(256), 0, st,
@@ -329,10 +397,18 @@ int launch_j(f2jk_ctx* ctx, const Plan& p, const TB* B, const double* const* D, 
     TIM(F2JK_PH_JGAMMA, false);
     TIM(F2JK_PH_JACC, true);
     hipLaunchKernelGGL((j_accum_kernel<TB, VEC, kEPT, ND, NT>), dim3(p.neg, p.npc2), dim3(kJThreads), 0,
-                       st, B, p.N, p.Q, gamma, rowoff, p.nvec, p.ppw2, jpart);
+                       st, B, p.N, p.Q, gamma, rowoff, elist, p.nvec, p.ppw2, jpart);
     F2JK_LAUNCHED("j_accum");
+    if (elist) {
+        // elements outside the list are exactly zero (B is zero there for every P)
+        const int64_t NN = (int64_t)p.N * p.N;
+        for (int d = 0; d < ND; ++d) {
+            hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)cdiv(NN, 256)), dim3(256), 0, st, J[d], NN);
+            F2JK_LAUNCHED("fill_zero");
+        }
+    }
     hipLaunchKernelGGL((j_reduce_kernel<VEC, ND>), dim3((unsigned)cdiv(p.nvec, 256)), dim3(256), 0,
-                       st, jpart, p.npc2, p.N, rowoff, p.nvec, jp);
+                       st, jpart, p.npc2, p.N, rowoff, elist, p.nvec, jp);
     F2JK_LAUNCHED("j_reduce");
     TIM(F2JK_PH_JACC, false);
     return F2JK_OK;
@@ -839,6 +915,8 @@ int f2jk_ctx_destroy(f2jk_ctx* ctx) {
     }
     for (int v = 0; v < 5; ++v)
         if (ctx->rowoff[v]) (void)hipFree(ctx->rowoff[v]);
+    free_jlist(ctx, F2JK_F64);
+    free_jlist(ctx, F2JK_F32);
     delete ctx;
     return F2JK_OK;
 }
@@ -868,6 +946,7 @@ int f2jk_set_B(f2jk_ctx* ctx, int dtype, const void* B, int64_t naux, int64_t nb
     // build the J element tables now so that builds allocate nothing
     int rc = ensure_rowoff(ctx, j_vec_for(dtype, nbf));
     if (rc) return rc;
+    if (ctx->j_sparse && (rc = build_jlist(ctx, dtype))) return rc;
     g_last_error.clear();
     return F2JK_OK;
 }
@@ -886,6 +965,16 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
         ctx->xchunk_bytes = (size_t)value;
     } else if (!strcmp(key, "j_nontemporal")) {
         ctx->j_nontemporal = value != 0;
+    } else if (!strcmp(key, "j_sparse")) {
+        ctx->j_sparse = value != 0;
+        for (int dt = 0; dt < 2; ++dt) {
+            if (ctx->j_sparse) {
+                int rc = build_jlist(ctx, dt);
+                if (rc) return rc;
+            } else {
+                free_jlist(ctx, dt);
+            }
+        }
     } else if (!strcmp(key, "ks_dma")) {
         if (value < -1 || value > 1) return fail(F2JK_EINVAL, "ks_dma must be -1, 0 or 1");
         ctx->ks_dma = (int)value;

@@ -111,10 +111,28 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
 // J pass 1: partial gamma per (element group, density, P).
 // grid = (n_element_groups, ceil(Q / ppw)), block = 256.
 // ---------------------------------------------------------------------------
+// Vector element e of the lower triangle: row m and first column col.  With
+// an element list (only the elements where B is nonzero for some P, packed
+// as m << 16 | col) e indexes the list; otherwise elements are numbered row
+// after row through rowoff.
+template <int VEC>
+__device__ __forceinline__ void j_element(const int* __restrict__ rowoff, const uint32_t* __restrict__ elist, int N,
+                                          int e, int& m, int& col) {
+    if (elist) {
+        const uint32_t x = elist[e];
+        m = (int)(x >> 16);
+        col = (int)(x & 0xffffu);
+    } else {
+        m = find_row(rowoff, N, e);
+        col = (e - rowoff[m]) * VEC;
+    }
+}
+
 template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __device__ __forceinline__ void j_gamma_kernel_body(
     const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
-    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart, int bx, int by) {
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart, int bx, int by,
+    const uint32_t* __restrict__ elist = nullptr) {
     __shared__ double red[kJThreads / kWave][kJFlush][ND];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int64_t NN = (int64_t)N * N;
@@ -130,8 +148,8 @@ __device__ __forceinline__ void j_gamma_kernel_body(
 #pragma unroll
             for (int v = 0; v < VEC; ++v) d[k][dd][v] = 0.0;
         if (e < nvec) {
-            const int m = find_row(rowoff, N, e);
-            const int col = (e - rowoff[m]) * VEC;
+            int m, col;
+            j_element<VEC>(rowoff, elist, N, e, m, col);
             off[k] = (int64_t)m * N + col;
 #pragma unroll
             for (int v = 0; v < VEC; ++v) {
@@ -218,8 +236,10 @@ __device__ __forceinline__ void j_gamma_kernel_body(
 template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __global__ __launch_bounds__(kJThreads) void j_gamma_kernel(
     const TB* __restrict__ B, int N, int Q, DPtrs<ND> D, int64_t ldd,
-    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ gpart) {
-    j_gamma_kernel_body<TB, VEC, EPT, ND, NT>(B, N, Q, D, ldd, rowoff, nvec, ppw, gpart, blockIdx.x, blockIdx.y);
+    const int* __restrict__ rowoff, const uint32_t* __restrict__ elist, int nvec, int ppw,
+    double* __restrict__ gpart) {
+    j_gamma_kernel_body<TB, VEC, EPT, ND, NT>(B, N, Q, D, ldd, rowoff, nvec, ppw, gpart, blockIdx.x, blockIdx.y,
+                                              elist);
 }
 
 
@@ -241,7 +261,8 @@ __global__ void j_gamma_reduce_kernel(const double* __restrict__ gpart, int ngro
 template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __device__ __forceinline__ void j_accum_kernel_body(
     const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
-    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart, int bx, int by) {
+    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart, int bx, int by,
+    const uint32_t* __restrict__ elist = nullptr) {
     const int tid = threadIdx.x;
     const int64_t NN = (int64_t)N * N;
     int64_t off[EPT];
@@ -252,8 +273,9 @@ __device__ __forceinline__ void j_accum_kernel_body(
         eidx[k] = e;
         off[k] = 0;
         if (e < nvec) {
-            const int m = find_row(rowoff, N, e);
-            off[k] = (int64_t)m * N + (int64_t)(e - rowoff[m]) * VEC;
+            int m, col;
+            j_element<VEC>(rowoff, elist, N, e, m, col);
+            off[k] = (int64_t)m * N + col;
         }
     }
     double acc[EPT][ND][VEC];
@@ -319,19 +341,22 @@ __device__ __forceinline__ void j_accum_kernel_body(
 template <typename TB, int VEC, int EPT, int ND, bool NT = false>
 __global__ __launch_bounds__(kJThreads) void j_accum_kernel(
     const TB* __restrict__ B, int N, int Q, const double* __restrict__ gamma,
-    const int* __restrict__ rowoff, int nvec, int ppw, double* __restrict__ jpart) {
-    j_accum_kernel_body<TB, VEC, EPT, ND, NT>(B, N, Q, gamma, rowoff, nvec, ppw, jpart, blockIdx.x, blockIdx.y);
+    const int* __restrict__ rowoff, const uint32_t* __restrict__ elist, int nvec, int ppw,
+    double* __restrict__ jpart) {
+    j_accum_kernel_body<TB, VEC, EPT, ND, NT>(B, N, Q, gamma, rowoff, nvec, ppw, jpart, blockIdx.x, blockIdx.y,
+                                              elist);
 }
 
 
 // J[dd][m][c] = J[dd][c][m] = sum_chunk jpart[chunk][dd][e][v]  (c = col+v <= m)
 template <int VEC, int ND>
 __global__ void j_reduce_kernel(const double* __restrict__ jpart, int nchunks, int N,
-                                const int* __restrict__ rowoff, int nvec, JPtrs<ND> J) {
+                                const int* __restrict__ rowoff, const uint32_t* __restrict__ elist, int nvec,
+                                JPtrs<ND> J) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nvec) return;
-    const int m = find_row(rowoff, N, e);
-    const int col = (e - rowoff[m]) * VEC;
+    int m, col;
+    j_element<VEC>(rowoff, elist, N, e, m, col);
 #pragma unroll
     for (int dd = 0; dd < ND; ++dd) {
         double s[VEC];
@@ -351,6 +376,33 @@ __global__ void j_reduce_kernel(const double* __restrict__ jpart, int nchunks, i
             }
         }
     }
+}
+
+// flag[e] = 1 when B[P][m][c] != 0 for some P and some c <= m of vector
+// element e (lower triangle, numbered row after row): the elements the J
+// passes have to read.  One thread per element, all P.
+template <typename TB, int VEC>
+__global__ void j_scan_kernel(const TB* __restrict__ B, int N, int Q, const int* __restrict__ rowoff, int nvec,
+                              unsigned char* __restrict__ flag) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nvec) return;
+    const int m = find_row(rowoff, N, e);
+    const int col = (e - rowoff[m]) * VEC;
+    const int64_t NN = (int64_t)N * N;
+    const TB* p = B + (int64_t)m * N + col;
+    int nz = 0;
+    for (int P = 0; P < Q && !nz; ++P) {
+        const Vec<TB, VEC> b = load_vec<TB, VEC>(p + P * NN);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) nz |= (col + v <= m) && (b.v[v] != TB(0));
+    }
+    flag[e] = (unsigned char)nz;
+}
+
+// x[i] = 0 for i < n
+__global__ void fill_zero_kernel(double* __restrict__ x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------

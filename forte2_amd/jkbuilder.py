@@ -48,9 +48,12 @@ class FockBuilder:
     precision : {"fp64", "fp32", "bf16x3"}
         Arithmetic of ``build_*`` calls (see include/f2jk.h).  fp32/bf16x3
         create an FP32 copy of B next to the FP64 one.
+    j_sparse : bool
+        J passes over the lower-triangle elements where B is nonzero for some
+        P only (B is scanned once per copy; its zero pattern must not change).
     """
 
-    def __init__(self, system=None, store_B_nPm=True, *, B=None, precision="fp64", device=None):
+    def __init__(self, system=None, store_B_nPm=True, *, B=None, precision="fp64", device=None, j_sparse=True):
         self.system = system
         self.store_B_nPm = store_B_nPm
         self.device = _device_of(device)
@@ -61,6 +64,7 @@ class FockBuilder:
         self._manual = set()  # launch-shape knobs set by hand
         self._tuned = None    # precision whose tuned launch shape is applied
         self.knobs = {}       # launch-shape knobs set on the context (by hand or from the table)
+        self.j_sparse = bool(j_sparse)
         self.timing = False
         self.last_timings = None
         self.last_launches = None
@@ -119,6 +123,11 @@ class FockBuilder:
             self._register(nat.F2JK_F64, self.B_Pmn)
             if self._B32 is not None:
                 self._register(nat.F2JK_F32, self._B32)
+            # J reads only the elements where B is nonzero for some P (B is
+            # fixed once formed; the scan happens here and when the FP32 copy
+            # is registered)
+            if self.j_sparse:
+                self._tune("j_sparse", 1)
         return self._ctx
 
     def _register(self, dtype, B):

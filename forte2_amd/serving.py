@@ -164,7 +164,10 @@ class SCFServer:
         system, M, T, V_sap = prepared
         system.device = self.device
         B = _df.device_B_from_host(M, T, self.device, system.nbf, getattr(system, "df_ortho_rtol", None))
-        fb = FockBuilder(system, B=B, device=self.device)
+        # J over all elements (j_sparse off): a job's J must not depend on
+        # whether a step ran grouped (the group reads every element) or alone,
+        # and the element scan would synchronise the device at admission
+        fb = FockBuilder(system, B=B, device=self.device, j_sparse=False)
         # launch shape of the job's size class (tuned for grouped builds when
         # batching); the X chunk is pinned so that the job's builds that run
         # alone keep the plan the grouped path was asked about

@@ -18,7 +18,8 @@
  *     given hipStream_t (NULL = default stream) and never synchronises.
  *   - Scratch memory is supplied by the caller (ws, ws_bytes); query its size
  *     with f2jk_workspace_size.  The library allocates device memory only in
- *     f2jk_ctx_create / f2jk_set_B (small index tables).
+ *     f2jk_ctx_create / f2jk_set_B / f2jk_set_tuning("j_sparse") (small
+ *     index tables).
  *   - Reductions are deterministic: a result depends only on the inputs and
  *     the context's shapes, never on concurrent work.
  *   - Return value: 0 on success, a negative f2jk_status otherwise; the
@@ -105,7 +106,14 @@ int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
  * even widths, coefficient tiles loaded straight into LDS); "ks_dma" (-1, 0,
  * 1; default -1 = with 256-wide tiles) loads the BF16x3 contraction's X
  * tiles straight into LDS instead of staging them through registers (the same
- * products, bitwise).
+ * products, bitwise); "j_sparse" (0/1, default 0): when 1, every registered B
+ * copy is scanned once (now, and at each later f2jk_set_B) for the
+ * lower-triangle vector elements where it is nonzero for some P, and the J
+ * passes read only those (density-fitted B of a molecule is exactly zero on
+ * its screened shell pairs: ~75% of the FP64 elements of (H2O)_64); J
+ * elsewhere is exactly zero, as in the full sum.  B's zero pattern must not
+ * change while registered (register it again to rescan).  The scan
+ * synchronises the device.
  * Set by the autotuner. */
 int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value);
 

@@ -235,3 +235,87 @@ def test_edge_shapes(dev):
             assert np.abs(K[k].cpu().numpy() - Kr[k]).max() <= TOL_FP64
         if noccs[0] == 0:
             assert float(J[0].abs().max()) == 0.0 and float(K[0].abs().max()) == 0.0
+
+
+def _screened(B, N, group=8, seed=3):
+    """B zeroed, for every P, on the (m, n) blocks of 'distant' function
+    groups (a symmetric random pattern of group x group blocks, about 70% of
+    them, diagonal blocks kept): the structure of a molecule's B on its
+    screened shell pairs.  Returns the mask of kept (m, n)."""
+    rng = np.random.default_rng(seed)
+    ng = -(-N // group)
+    keep = rng.random((ng, ng)) < 0.3
+    keep = keep | keep.T | np.eye(ng, dtype=bool)
+    mask = np.kron(keep, np.ones((group, group), dtype=bool))[:N, :N]
+    B.mul_(torch.from_numpy(mask).to(B.device, B.dtype))
+    return mask
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize(
+    "Q,N,noccs",
+    [
+        (300, 132, [17, 13]),     # two densities, FP32 J in 4-wide elements
+        (200, 96, [20]),
+        (120, 31, [6]),           # odd nbf: 1-wide elements
+    ],
+)
+def test_j_over_nonzero_elements(dev, prec, Q, N, noccs):
+    # J through the list of lower-triangle elements where B is nonzero for some
+    # P equals J over every element (FP64 to rounding of the gamma sums, the
+    # precision forms to their J bound) and the oracle, and is exactly zero
+    # where B is zero for every P.  K does not use the list: bitwise equal.
+    B, Cs, _, Ch = _case(dev, Q, N, noccs, seed=41)
+    mask = _screened(B, N)
+    Bh = B.cpu().numpy()
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    out = {}
+    for sparse in (False, True):
+        fb = FockBuilder(B=B, device=dev, j_sparse=sparse)
+        out[sparse] = fb.build_JK(Cs, precision=prec)
+        # build_J from densities (f2jk_build_j) through the same list
+        out[(sparse, "D")] = fb.build_J([c @ c.T for c in Cs], precision=prec)
+        fb.release()
+    for k in range(len(noccs)):
+        Js, Jd = out[True][0][k], out[False][0][k]
+        jr = Jr[k]
+        err = float(np.abs(Js.cpu().numpy() - jr).max())
+        rel = err / float(np.abs(jr).max())
+        print(f"   {prec} J over nonzero elements, dens {k}: |dJ|={err:.2e} rel {rel:.2e}", flush=True)
+        if prec == "fp64":
+            assert err <= TOL_FP64
+            assert float((Js - Jd).abs().max()) <= 1e-12 * float(Jd.abs().max())
+        else:
+            assert rel < 1e-6
+        assert bool((Js.cpu().numpy()[~mask] == 0.0).all())
+        assert torch.equal(out[True][1][k], out[False][1][k])
+        Jds = out[(True, "D")][k]
+        assert float((Jds - Js).abs().max()) <= 1e-12 * float(Js.abs().max()) + (0.0 if prec == "fp64" else 1e-7)
+
+
+def test_j_over_nonzero_elements_real_h2o16(dev):
+    # the (H2O)_16 cluster of the SCF tests (cc-pVDZ / cc-pVTZ-JKFIT, seed 16,
+    # 9 A box): its B formed on the device is exactly zero on the screened
+    # shell pairs, and J read through the nonzero-element list matches the
+    # oracle on the same B to 1e-10
+    from forte2_amd import geometry as geo
+    from forte2_amd.system import System
+
+    s = System(geo.water_cluster(16, 9.0, seed=16), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    fb = s.fock_builder
+    B = fb.B_Pmn
+    N = s.nbf
+    C = orthonormal_C(N, 80, seed=16, device=dev)
+    J, K = fb.build_JK([C])
+    Bh = B.cpu().numpy()
+    zero = ~(Bh != 0.0).any(axis=0)
+    print(f"   (H2O)_16: B zero for every P on {zero.mean():.1%} of (m, n)", flush=True)
+    Jr, Kr = ojk.build_JK(Bh, [C.cpu().numpy()])
+    ej = float(np.abs(J[0].cpu().numpy() - Jr[0]).max())
+    ek = float(np.abs(K[0].cpu().numpy() - Kr[0]).max())
+    print(f"   |dJ|={ej:.2e} |dK|={ek:.2e}", flush=True)
+    assert zero.mean() > 0.2
+    assert ej <= TOL_FP64 and ek <= TOL_FP64
+    assert bool((J[0].cpu().numpy()[zero] == 0.0).all())
+    fb.release()
+    s.fock_builder = None
