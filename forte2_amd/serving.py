@@ -437,6 +437,16 @@ class SCFServer:
     def summary(self):
         ok = [r for r in self.results.values() if r.error is None and r.converged]
         lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+        # where a request's latency goes: waiting for a slot or HBM (arrival ->
+        # admission), host integrals and B formation (-> start on the device),
+        # the SCF on the device (-> completion)
+        parts = {
+            "queue": np.array([r.admitted - r.arrival for r in ok]) if ok else np.zeros(1),
+            "prep": np.array([r.started - r.admitted for r in ok]) if ok else np.zeros(1),
+            "device": np.array([r.finished - r.started for r in ok]) if ok else np.zeros(1),
+        }
+        split = {f"{k}_{stat}_s": float(f(v)) for k, v in parts.items()
+                 for stat, f in (("mean", np.mean), ("p99", lambda x: np.percentile(x, 99)))}
         return {
             "requests": len(self.results),
             "completed": len(ok),
@@ -451,6 +461,7 @@ class SCFServer:
             "group_declines": self.group_declines,
             "group_failures": self.group_failures,
             "build_failures": self.build_failures,
+            **split,
         }
 
 
