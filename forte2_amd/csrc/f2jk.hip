@@ -290,6 +290,9 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
         const size_t per_p = (size_t)N * p.OPAD * p.tk;
         int64_t pc = (int64_t)(xchunk_bytes / std::max<size_t>(per_p, 1));
         pc = std::max<int64_t>(1, std::min<int64_t>(pc, Q));
+        // as many chunks as the cap needs, of balanced size: no short last
+        // chunk whose launches would be all set-up and Kpart traffic
+        pc = cdiv(Q, cdiv(Q, pc));
         p.Pc = (int)pc;
         p.nchunks = (int)cdiv(Q, pc);
         // BF16x3 single-job builds of large nbf contract in 256-wide tiles
@@ -299,8 +302,11 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
             p.ks_tile = ctx->ks_tile ? ctx->ks_tile : (N >= 768 ? 256 : 128);
         p.ntile = (int)cdiv(N, p.ks_tile);
         p.nlower = p.ntile * (p.ntile + 1) / 2;
+        // splits of the P range: at most wg_target workgroups per launch
+        // (rounded down, so that a launch fills whole rounds of the CUs
+        // instead of starting a nearly empty last one)
         const int64_t wg_target = p.ks_tile == 256 ? target_blocks / 4 : target_blocks / 2;
-        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, cdiv(wg_target, p.nlower)));
+        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(pc, wg_target / p.nlower));
         if (precision == F2JK_PREC_BF16X3) {
             // single-job builds of large nbf run the tall (256-row) half-transform
             p.kx_rows = 128;
