@@ -98,6 +98,44 @@ def test_noalloc_matches_build_jk_on_two_calls(dev, prec):
         del B, B32, Cs
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_noalloc_with_j_over_nonzero_elements(dev, prec):
+    # j_sparse on: the element list is the context's (made by set_tuning, not
+    # by the build), so the no-alloc build reads the same list; B zeroed on a
+    # symmetric block pattern, as a molecule's B on its screened pairs
+    import numpy as np
+
+    lib = nat.jk_lib()
+    Q, N, o = 200, 96, 20
+    print(f"[no-alloc] {prec} j_sparse Q={Q} N={N} o={o}", flush=True)
+    B, Cs = _case(dev, Q, N, o, 11)
+    rng = np.random.default_rng(5)
+    keep = rng.random((12, 12)) < 0.3
+    keep = keep | keep.T | np.eye(12, dtype=bool)
+    mask = np.kron(keep, np.ones((8, 8), dtype=bool))
+    B.mul_(torch.from_numpy(mask).to(dev, B.dtype))
+    B32 = B.to(torch.float32) if prec != "fp64" else None
+    h = _ctx(dev, B, B32)
+    try:
+        nat.jk_check(lib.f2jk_set_tuning(h, b"j_sparse", 1), "j_sparse")
+        sz = ctypes.c_size_t()
+        nat.jk_check(lib.f2jk_workspace_size(h, 1, nat.i64_array([o]), nat.PREC[prec],
+                                             nat.F2JK_WANT_J | nat.F2JK_WANT_K, ctypes.byref(sz)), "ws")
+        Jn, Kn = _build(h, Cs, prec, True, dev)
+        Jw, Kw = _build(h, Cs, prec, False, dev)
+        zero = torch.from_numpy(~mask).to(dev)
+        assert float(Jn[0][zero].abs().max()) == 0.0
+        for a, b in zip(Jn + Kn, Jw + Kw):
+            if sz.value <= nat.F2JK_NOALLOC_SCRATCH_BYTES:  # the same launch shapes
+                assert torch.equal(a, b)
+            elif prec == "fp64":
+                assert float((a - b).abs().max()) <= 1e-10
+            else:
+                assert float((a - b).abs().max()) <= 2e-5 * float(b.abs().max())
+    finally:
+        lib.f2jk_ctx_destroy(h)
+
+
 def test_noalloc_refuses_a_build_beyond_its_scratch(dev):
     lib = nat.jk_lib()
     # nbf 1536 ((H2O)_64 size) needs more than the array holds even at one P
