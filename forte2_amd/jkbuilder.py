@@ -97,11 +97,20 @@ class FockBuilder:
     def B_Pmn(self):
         """B(P|mn) as a device tensor (built lazily from the system)."""
         if self._B64 is None:
-            from .df import build_B_density_fitting
-
             self.B_stats = {}
-            self._set_B(build_B_density_fitting(self.system, device=self.device, progress=self.progress,
-                                                stats=self.B_stats))
+            if getattr(self.system, "cholesky_tei", False):
+                # Cholesky vectors of (mn|ls); naux is their number
+                # (forte2/jkbuilder/jkbuilder.py:66-70, 90-117)
+                from .cholesky import build_B_cholesky
+
+                self._set_B(build_B_cholesky(self.system, self.system.cholesky_tol, self.device,
+                                             stats=self.B_stats))
+                self.system.naux = self.naux
+            else:
+                from .df import build_B_density_fitting
+
+                self._set_B(build_B_density_fitting(self.system, device=self.device, progress=self.progress,
+                                                    stats=self.B_stats))
         return self._B64
 
     def B_fp32(self):

@@ -28,7 +28,16 @@ def canonical_orth(S, rtol=1e-8):
 class System:
     def __init__(self, xyz, basis_set, auxiliary_basis_set=None, unit="angstrom",
                  overlap_ortho_rtol=1e-8, df_ortho_rtol=None, device=None, jk_mem_thres_mb=None,
-                 integral_backend="auto"):
+                 integral_backend="auto", cholesky_tei=False, cholesky_tol=1e-6):
+        # B from a pivoted Cholesky decomposition of (mn|ls) instead of
+        # density fitting: any auxiliary basis is ignored and naux is the rank,
+        # known once B is built (forte2/system/system.py:123-124, 159-160, 341-352)
+        if cholesky_tol < 0:
+            raise ValueError("cholesky_tol must be non-negative.")
+        self.cholesky_tei = bool(cholesky_tei)
+        self.cholesky_tol = float(cholesky_tol)
+        if self.cholesky_tei:
+            auxiliary_basis_set = None
         self.xyz = xyz
         self.unit = unit
         self.atoms = _geom.parse_geometry(xyz, unit) if isinstance(xyz, str) else list(xyz)
@@ -39,7 +48,10 @@ class System:
         self.auxiliary_basis = (_basis.build_basis(auxiliary_basis_set, self.atoms)
                                 if auxiliary_basis_set is not None else None)
         self.nbf = self.basis.size
-        self.naux = self.auxiliary_basis.size if self.auxiliary_basis is not None else 0
+        if self.cholesky_tei:
+            self.naux = None
+        else:
+            self.naux = self.auxiliary_basis.size if self.auxiliary_basis is not None else 0
         self.df_ortho_rtol = df_ortho_rtol
         self.overlap_ortho_rtol = overlap_ortho_rtol
         self.nuclear_repulsion = _ints.nuclear_repulsion(self.atoms)
@@ -103,6 +115,8 @@ class System:
         exceed jk_mem_thres_mb (MB of HBM) (forte2/system/system.py:405-434)."""
         from .jkbuilder import FockBuilder
 
+        if self.jk_mem_thres_mb is not None and self.cholesky_tei:
+            raise ValueError("If cholesky_tei is True, jk_mem_thres_mb must be None.")
         if self.jk_mem_thres_mb is None:
             return FockBuilder(self, device=self.device)
         if self.auxiliary_basis is None:

@@ -1,0 +1,79 @@
+"""The Cholesky-TEI B source on the device (forte2_amd.cholesky; the
+reference's System(cholesky_tei=True), jkbuilder.py:90-117).
+
+* tests/helpers/test_cholesky.py on the device path: the device factor of the
+  water cc-pVDZ (mn|ls) at tol 1e-14 reproduces it to 1e-10 (Frobenius), with
+  dpstrf's rank (the oracle), and B is exactly symmetric in (m, n).
+* tests/scf/test_rhf.py:47-59: the device RHF with Cholesky B at tol 1e-10
+  gives the golden -76.021769351262 at approx(abs=5e-8), and the oracle RHF on
+  the oracle's (dpstrf) B to <= 1e-10 Eh.
+* J/K of the device builder on that B equal the oracle's on the same B to
+  <= 1e-10, and the precision form ends within 1e-8 Eh of FP64.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from forte2_amd import integrals as ints
+from forte2_amd.scf import RHF
+from forte2_amd.system import System
+from oracle import cholesky as ochol
+from oracle import jk as ojk
+from oracle import scf as oscf
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+ECHOL = GOLD["energies"]["water_dz_rhf_cholesky"]
+
+WATER = """O            0.000000000000     0.000000000000    -0.061664597388
+H            0.000000000000    -0.711620616369     0.489330954643
+H            0.000000000000     0.711620616369     0.489330954643"""
+
+
+def _system(dev, tol):
+    return System(WATER, "cc-pvdz", cholesky_tei=True, cholesky_tol=tol, device=dev)
+
+
+def test_device_cholesky_factor_of_the_water_eri(dev):
+    s = _system(dev, 1e-14)
+    B = s.fock_builder.B_Pmn
+    N = s.nbf
+    V = ints.coulomb_4c(s.basis).reshape(N * N, N * N)
+    Bh = B.cpu().numpy()
+    print(f"[cholesky-gpu] tol 1e-14: rank {Bh.shape[0]} of {N * N}", flush=True)
+    assert s.naux == Bh.shape[0]
+    assert np.array_equal(Bh, Bh.transpose(0, 2, 1))
+    Bf = Bh.reshape(Bh.shape[0], N * N)
+    assert np.linalg.norm(Bf.T @ Bf - V) < 1e-10
+    assert Bh.shape[0] == ochol.cholesky_wrapper(V, 1e-14).shape[0]
+    s.fock_builder.release()
+
+
+def test_device_rhf_with_cholesky_B_golden_and_oracle(dev):
+    tol = ECHOL["cholesky_tol"]
+    s = _system(dev, tol)
+    r = RHF(charge=0, e_tol=1e-10)(s).run()
+    N = s.nbf
+    Bo = ochol.build_B_cholesky(ints.coulomb_4c(s.basis), tol)
+    ro = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(Bo), s.Zsum,
+                      X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+    print(f"[cholesky-gpu] tol {tol}: naux {s.naux} (oracle {Bo.shape[0]}) device {r.E:.12f} "
+          f"oracle {ro.E:.12f} golden {ECHOL['E']:.12f}", flush=True)
+    assert r.converged
+    assert r.E == pytest.approx(ECHOL["E"], abs=5e-8)
+    assert abs(r.E - ro.E) <= 1e-10
+    # J/K through the C ABI on the device's B against the oracle on the same B
+    Bd = s.fock_builder.B_Pmn.cpu().numpy()
+    C = r.C[0][:, : s.Zsum // 2].cpu().numpy()
+    J, K = s.fock_builder.build_JK([C])
+    Jo, Ko = ojk.build_JK(Bd, [C])
+    assert np.abs(J[0] - Jo[0]).max() <= 1e-10 and np.abs(K[0] - Ko[0]).max() <= 1e-10
+    # the precision form on the same B
+    mix = RHF(charge=0, e_tol=1e-10, precision="bf16x3")(s).run()
+    assert mix.converged and abs(mix.E - r.E) <= 1e-8
+    assert N == 24
+    s.fock_builder.release()

@@ -1,0 +1,122 @@
+"""The Cholesky-TEI B source on the CPU: the oracle (oracle/cholesky.py,
+forte2/helpers/matrix_functions.py:239-270 and jkbuilder.py:90-117) against
+the reference's own checks and golden energy, and the device algorithm
+(forte2_amd.cholesky.pivoted_cholesky, run here on CPU tensors) against the
+oracle.
+
+* tests/helpers/test_cholesky.py: B^T B = V to 1e-10 for the water cc-pVDZ
+  (mn|ls) at tol 1e-14, and for a random positive semi-definite matrix whose
+  last row and column are zero (plain Cholesky fails on it).
+* tests/scf/test_rhf.py:47-59: RHF with Cholesky B at tol 1e-10,
+  -76.021769351262 at approx(abs=5e-8).
+* The device algorithm reaches the rank and residual V - B^T B of dpstrf: at
+  tol 1e-10 the same rank and the same RHF energy to 1e-10 Eh.
+* System validation as the reference (system.py:159-160, 405-407).
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from forte2_amd import integrals as ints
+from forte2_amd.cholesky import pivoted_cholesky
+from forte2_amd.system import System
+from oracle import cholesky as ochol
+from oracle import scf as oscf
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
+ECHOL = GOLD["energies"]["water_dz_rhf_cholesky"]
+
+WATER = """O            0.000000000000     0.000000000000    -0.061664597388
+H            0.000000000000    -0.711620616369     0.489330954643
+H            0.000000000000     0.711620616369     0.489330954643"""
+
+
+def _water_V():
+    s = System(WATER, "cc-pvdz", cholesky_tei=True, cholesky_tol=ECHOL["cholesky_tol"])
+    N = s.nbf
+    return s, ints.coulomb_4c(s.basis).reshape(N * N, N * N)
+
+
+def _random_psd():
+    g = np.random.default_rng(42)
+    M = g.random((50, 50))
+    M = M @ M.T
+    M[:, -1] = 0
+    M[-1, :] = 0
+    return M
+
+
+def test_oracle_factor_of_a_singular_psd_matrix():
+    M = _random_psd()
+    B = ochol.cholesky_wrapper(M, tol=1e-14)
+    assert np.linalg.norm(B.T @ B - M) < 1e-10
+    assert B.shape[0] == 49
+    with pytest.raises(np.linalg.LinAlgError):
+        np.linalg.cholesky(M)
+
+
+def test_oracle_factor_of_the_water_eri():
+    s, V = _water_V()
+    B = ochol.cholesky_wrapper(V, tol=1e-14)
+    assert np.linalg.norm(B.T @ B - V) < 1e-10
+
+
+def _rhf(s, B):
+    return oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                        X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10)
+
+
+def test_oracle_rhf_with_cholesky_B_golden():
+    s, V = _water_V()
+    N = s.nbf
+    B = ochol.build_B_cholesky(V.reshape(N, N, N, N), ECHOL["cholesky_tol"])
+    r = _rhf(s, B)
+    assert r.E == pytest.approx(ECHOL["E"], abs=5e-8)
+
+
+@pytest.mark.parametrize("tol", [1e-14, 1e-10, 1e-6, -1.0])
+def test_device_algorithm_matches_dpstrf(tol):
+    s, V = _water_V()
+    L = pivoted_cholesky(torch.from_numpy(V), tol, block=16).numpy()
+    B = ochol.cholesky_wrapper(V, tol)
+    assert L.shape == B.shape  # same rank
+    # The residual V - L^T L is positive semi-definite with a diagonal <= the
+    # stopping value, so no element exceeds it (rounding aside); the vectors
+    # themselves may differ where symmetry-equivalent pairs tie for the pivot
+    # (the two H atoms), so the factors are compared through V.
+    stop = tol if tol >= 0 else V.shape[0] * np.finfo(float).eps * np.diag(V).max()
+    res = np.abs(L.T @ L - V).max()
+    assert res <= stop + 1e-13
+    assert res <= 1.01 * np.abs(B.T @ B - V).max() + 1e-14
+
+
+def test_device_algorithm_on_a_singular_psd_matrix():
+    M = _random_psd()
+    L = pivoted_cholesky(torch.from_numpy(M), 1e-14, block=7).numpy()
+    assert L.shape[0] == 49
+    assert np.linalg.norm(L.T @ L - M) < 1e-10
+
+
+def test_device_algorithm_rhf_equals_oracle():
+    s, V = _water_V()
+    N = s.nbf
+    tol = ECHOL["cholesky_tol"]
+    L = pivoted_cholesky(torch.from_numpy(V), tol).numpy().reshape(-1, N, N)
+    L = 0.5 * (L + L.transpose(0, 2, 1))
+    B = ochol.build_B_cholesky(V.reshape(N, N, N, N), tol)
+    assert L.shape == B.shape
+    assert abs(_rhf(s, L).E - _rhf(s, B).E) <= 1e-10
+
+
+def test_system_validation():
+    with pytest.raises(ValueError, match="non-negative"):
+        System(WATER, "cc-pvdz", cholesky_tei=True, cholesky_tol=-1.0)
+    s = System(WATER, "cc-pvdz", "cc-pvtz-jkfit", cholesky_tei=True)
+    assert s.auxiliary_basis is None and s.naux is None  # the fit set is ignored
+    s = System(WATER, "cc-pvdz", cholesky_tei=True, jk_mem_thres_mb=100.0)
+    with pytest.raises(ValueError, match="jk_mem_thres_mb must be None"):
+        s.fock_builder
