@@ -20,6 +20,10 @@ H            0.000000000000    -0.711620616369     0.489330954643
 H            0.000000000000     0.711620616369     0.489330954643"""
 
 
+# tests/scf/test_rhf_long.py:12-29
+NAPHTHALENE = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "naphthalene.xyz")).read()
+
+
 def _run(xyz, basis, aux, unit="angstrom", ms=None, guess="sap", **kw):
     s = System(xyz, basis, aux, unit=unit)
     B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
@@ -43,6 +47,8 @@ def _run(xyz, basis, aux, unit="angstrom", ms=None, guess="sap", **kw):
         ("H 0.0 0.0 0.0\nF 0.0 0.0 2.0", "cc-pvdz", "cc-pvtz-jkfit", "bohr", EREF["hf_dz_rhf"]["E"]),
         # tests/scf/test_rhf.py:12-22
         (WATER, "cc-pvqz", "cc-pvqz-jkfit", "angstrom", EREF["water_qz_rhf"]["E"]),
+        # tests/scf/test_rhf_long.py:8-36 (C10H8, nbf 412, naux 1030; marked slow there)
+        (NAPHTHALENE, "cc-pvtz", "cc-pvtz-jkfit", "angstrom", EREF["naphthalene_tz_rhf"]["E"]),
     ],
 )
 def test_rhf_golden(xyz, basis, aux, unit, eref):

@@ -84,6 +84,26 @@ def test_rhf_h2o16_plain_form(dev):
     assert abs(scf.E - ref.E) <= 1e-10
 
 
+def test_rhf_naphthalene_tz_golden(dev):
+    # tests/scf/test_rhf_long.py:8-36 (C10H8 cc-pVTZ/cc-pVTZ-JKFIT, nbf 412,
+    # naux 1030; a slow test there): device FP64 against the golden energy
+    # and the oracle, the precision form against FP64
+    xyz = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "naphthalene.xyz")).read()
+    print("[scf-gpu] C10H8 cc-pVTZ/cc-pVTZ-JKFIT RHF", flush=True)
+    s = System(xyz, "cc-pvtz", "cc-pvtz-jkfit", device=dev)
+    assert (s.nbf, s.naux) == (412, 1030)
+    scf = RHF(charge=0, e_tol=1e-10)(s).run()
+    ref, _ = _oracle(s)
+    mix = RHF(charge=0, e_tol=1e-10, precision="bf16x3")(s).run()
+    print(f"   device {scf.E:.12f} oracle {ref.E:.12f} golden {EREF['naphthalene_tz_rhf']['E']:.12f} "
+          f"bf16x3 {mix.E:.12f} iters {scf.iter}/{ref.iter}/{mix.iter}", flush=True)
+    assert scf.converged and mix.converged
+    assert scf.E == pytest.approx(EREF["naphthalene_tz_rhf"]["E"], abs=5e-8)
+    assert abs(scf.E - ref.E) <= 1e-10
+    assert abs(mix.E - scf.E) <= 1e-8
+    s.fock_builder.release()
+
+
 def test_rhf_level_shift_golden(dev):
     # tests/scf/test_rhf.py:62-79: formamide, level shift 0.5, no DIIS
     print("[scf-gpu] formamide cc-pVTZ/cc-pVTZ-JKFIT RHF, level shift 0.5", flush=True)
