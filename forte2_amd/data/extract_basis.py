@@ -12,7 +12,7 @@ import os
 import sys
 
 SETS = ["sto-3g", "cc-pvdz", "cc-pvtz", "cc-pvqz", "cc-pvtz-jkfit", "cc-pvqz-jkfit",
-        "def2-universal-jkfit", "sap_helfem_large"]
+        "def2-universal-jkfit", "sap_helfem_large", "aug-cc-pvdz"]
 ELEMENTS = ["1", "3", "6", "7", "8", "9", "10", "17"]  # H Li C N O F Ne Cl
 
 

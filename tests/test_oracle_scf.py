@@ -56,6 +56,32 @@ def test_rhf_golden(xyz, basis, aux, unit, eref):
     assert r.E == pytest.approx(eref, abs=5e-8)
 
 
+def test_rhf_linear_dependence_golden():
+    # tests/scf/test_lindep.py:9-30: H10 chain (1 bohr apart) in aug-cc-pVDZ,
+    # cond(S) > 1e14, overlap_ortho_rtol 2e-7 keeps 79 of 90 orbitals
+    g = EREF["h10_augdz_rhf_lindep"]
+    s = System("\n".join(f"H 0 0 {i}" for i in range(10)), "aug-cc-pvdz", "cc-pvqz-jkfit", unit="bohr",
+               overlap_ortho_rtol=g["overlap_ortho_rtol"])
+    assert np.linalg.cond(s.ints_overlap()) > 1e14
+    assert (s.nbf, s.nmo) == (g["nbf"], g["nmo"])
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    r = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                     X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-10, d_tol=1e-8)
+    assert r.E == pytest.approx(g["E"], abs=5e-8)
+
+
+def test_rhf_zero_virtuals_and_zero_electrons():
+    # tests/scf/test_rhf.py:25-44: Ne in STO-3G (every orbital occupied) and
+    # H2 with charge +2 (no electron: E = nuclear repulsion)
+    r = _run("Ne 0 0 0", "sto-3g", "def2-universal-jkfit")
+    assert r.E == pytest.approx(EREF["ne_sto3g_rhf_zero_virtuals"]["E"], abs=5e-8)
+    s = System("H 0 0 0\nH 0 0 1.0", "cc-pvqz", "cc-pvqz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    r = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - 2,
+                     X=s.Xorth, V_sap=s.sap_potential())
+    assert r.E == pytest.approx(s.nuclear_repulsion, abs=5e-8)
+
+
 def test_final_iteration_moves_to_the_new_fock():
     # scf_base.py:214-221: after convergence the new, non-extrapolated Fock
     # matrix is diagonalised once more and rebuilt.  Diagonalising the DIIS

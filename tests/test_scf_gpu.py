@@ -31,10 +31,10 @@ E_WATER_QZ_UHF = EREF["water_qz_uhf_triplet"]["E"]     # tests/scf/test_uhf.py:1
 S2_WATER_QZ_UHF = EREF["water_qz_uhf_triplet"]["S2"]
 
 
-def _oracle(s, ms=None, charge=0, e_tol=1e-10):
+def _oracle(s, ms=None, charge=0, e_tol=1e-10, d_tol=1e-6):
     B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
     args = (s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - charge)
-    kw = dict(X=s.Xorth, V_sap=s.sap_potential(), e_tol=e_tol)
+    kw = dict(X=s.Xorth, V_sap=s.sap_potential(), e_tol=e_tol, d_tol=d_tol)
     if ms is None:
         return oscf.run_rhf(*args, **kw), B
     return oscf.run_uhf(*args, ms, **kw), B
@@ -102,6 +102,34 @@ def test_rhf_naphthalene_tz_golden(dev):
     assert abs(scf.E - ref.E) <= 1e-10
     assert abs(mix.E - scf.E) <= 1e-8
     s.fock_builder.release()
+
+
+def test_rhf_linear_dependence_golden(dev):
+    # tests/scf/test_lindep.py:9-30: cond(S) > 1e14, 79 of 90 orbitals kept
+    # (a rectangular orthogonaliser on the device), tight d_tol
+    g = EREF["h10_augdz_rhf_lindep"]
+    print("[scf-gpu] H10 chain aug-cc-pVDZ/cc-pVQZ-JKFIT RHF, linear dependence", flush=True)
+    s = System("\n".join(f"H 0 0 {i}" for i in range(10)), "aug-cc-pvdz", "cc-pvqz-jkfit", unit="bohr",
+               overlap_ortho_rtol=g["overlap_ortho_rtol"], device=dev)
+    assert (s.nbf, s.nmo) == (g["nbf"], g["nmo"])
+    scf = RHF(charge=0, e_tol=1e-10, d_tol=1e-8)(s).run()
+    ref, _ = _oracle(s, d_tol=1e-8)
+    print(f"   device {scf.E:.12f} oracle {ref.E:.12f} golden {g['E']:.12f}", flush=True)
+    assert scf.converged
+    assert scf.E == pytest.approx(g["E"], abs=5e-8)
+    assert abs(scf.E - ref.E) <= 1e-10
+
+
+def test_rhf_zero_virtuals_and_zero_electrons(dev):
+    # tests/scf/test_rhf.py:25-44
+    s = System("Ne 0 0 0", "sto-3g", "def2-universal-jkfit", device=dev)
+    scf = RHF(charge=0)(s).run()
+    assert scf.E == pytest.approx(EREF["ne_sto3g_rhf_zero_virtuals"]["E"], abs=5e-8)
+    s = System("H 0 0 0\nH 0 0 1.0", "cc-pvqz", "cc-pvqz-jkfit", device=dev)
+    scf = RHF(charge=2)(s).run()
+    print(f"[scf-gpu] Ne STO-3G {EREF['ne_sto3g_rhf_zero_virtuals']['E']:.12f}; H2++ {scf.E:.12f} "
+          f"(Vnn {s.nuclear_repulsion:.12f})", flush=True)
+    assert scf.E == pytest.approx(s.nuclear_repulsion, abs=5e-8)
 
 
 def test_rhf_level_shift_golden(dev):
