@@ -77,3 +77,13 @@ def test_device_rhf_with_cholesky_B_golden_and_oracle(dev):
     assert mix.converged and abs(mix.E - r.E) <= 1e-8
     assert N == 24
     s.fock_builder.release()
+
+
+def test_device_rhf_with_cholesky_B_n2_golden(dev):
+    # tests/mcopt/test_casscf_cholesky_n2.py:5-34 (its RHF energy)
+    g = GOLD["energies"]["n2_dz_rhf_cholesky"]
+    s = System("N 0 0 0\nN 0 0 1.120", "cc-pvdz", cholesky_tei=True, cholesky_tol=g["cholesky_tol"], device=dev)
+    r = RHF(charge=0, e_tol=1e-12)(s).run()
+    print(f"[cholesky-gpu] N2: naux {s.naux} device {r.E:.12f} golden {g['E']:.12f}", flush=True)
+    assert r.converged and r.E == pytest.approx(g["E"], abs=5e-8)
+    s.fock_builder.release()

@@ -78,6 +78,17 @@ def test_oracle_rhf_with_cholesky_B_golden():
     assert r.E == pytest.approx(ECHOL["E"], abs=5e-8)
 
 
+def test_oracle_rhf_with_cholesky_B_n2_golden():
+    # tests/mcopt/test_casscf_cholesky_n2.py:5-34 (its RHF energy; the
+    # System there also uses point-group symmetry, which this path does not)
+    g = GOLD["energies"]["n2_dz_rhf_cholesky"]
+    s = System("N 0 0 0\nN 0 0 1.120", "cc-pvdz", cholesky_tei=True, cholesky_tol=g["cholesky_tol"])
+    B = ochol.build_B_cholesky(ints.coulomb_4c(s.basis), g["cholesky_tol"])
+    r = oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum,
+                     X=s.Xorth, V_sap=s.sap_potential(), e_tol=1e-12)
+    assert r.E == pytest.approx(g["E"], abs=5e-8)
+
+
 @pytest.mark.parametrize("tol", [1e-14, 1e-10, 1e-6, -1.0])
 def test_device_algorithm_matches_dpstrf(tol):
     s, V = _water_V()

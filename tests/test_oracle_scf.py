@@ -12,6 +12,8 @@ from forte2_amd.system import System
 from oracle import jk as ojk
 from oracle import scf as oscf
 
+from golden_systems import MORE_RHF
+
 GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
 EREF = GOLD["energies"]
 
@@ -54,6 +56,14 @@ def _run(xyz, basis, aux, unit="angstrom", ms=None, guess="sap", **kw):
 def test_rhf_golden(xyz, basis, aux, unit, eref):
     r = _run(xyz, basis, aux, unit, e_tol=1e-10)
     assert r.E == pytest.approx(eref, abs=5e-8)
+
+
+
+
+@pytest.mark.parametrize("xyz,basis,aux,unit,key,kw", MORE_RHF, ids=[c[4] for c in MORE_RHF])
+def test_more_rhf_golden(xyz, basis, aux, unit, key, kw):
+    r = _run(xyz, basis, aux, unit, **kw)
+    assert r.E == pytest.approx(EREF[key]["E"], abs=5e-8)
 
 
 def test_rhf_linear_dependence_golden():

@@ -18,6 +18,8 @@ from forte2_amd.system import System
 from oracle import jk as ojk
 from oracle import scf as oscf
 
+from golden_systems import MORE_RHF
+
 GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_values.json")))
 EREF = GOLD["energies"]
 
@@ -102,6 +104,16 @@ def test_rhf_naphthalene_tz_golden(dev):
     assert abs(scf.E - ref.E) <= 1e-10
     assert abs(mix.E - scf.E) <= 1e-8
     s.fock_builder.release()
+
+
+@pytest.mark.parametrize("xyz,basis,aux,unit,key,kw", MORE_RHF, ids=[c[4] for c in MORE_RHF])
+def test_more_rhf_golden(dev, xyz, basis, aux, unit, key, kw):
+    # RHF energies asserted by the reference's DSRG and CASSCF tests
+    s = System(xyz, basis, aux, unit=unit, device=dev)
+    scf = RHF(charge=0, **kw)(s).run()
+    print(f"[scf-gpu] {key}: device {scf.E:.12f} golden {EREF[key]['E']:.12f}", flush=True)
+    assert scf.converged
+    assert scf.E == pytest.approx(EREF[key]["E"], abs=5e-8)
 
 
 def test_rhf_linear_dependence_golden(dev):
