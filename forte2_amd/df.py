@@ -21,7 +21,12 @@ def metric_factor(M, df_ortho_rtol=None):
     """(kind, factor) with B = factor^-1 T ("chol", L) or B = factor T ("invsqrt", X)."""
     if df_ortho_rtol is None:
         L, info = torch.linalg.cholesky_ex(M)
-        if int(info) != 0:
+        # A pivot at the rounding level of the largest diagonal element counts
+        # as zero: the LAPACK test (pivot <= 0) leaves an exactly singular
+        # metric, such as two identical fitting functions, to the rounding
+        # order of the factorisation, which differs between libraries.
+        tiny = torch.finfo(M.dtype).eps * M.diagonal().max()
+        if int(info) != 0 or bool((torch.diagonal(L) ** 2 <= tiny).any()):
             raise ValueError(
                 "Density fitting Coulomb metric (P|Q) is not positive definite.\n"
                 "Please set df_ortho_rtol to a small positive value to orthogonalize the metric.")

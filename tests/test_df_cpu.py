@@ -38,3 +38,26 @@ def test_packed_host_device_split_matches_oracle(w4):
     Be = df.device_B_from_host(Mh, packed, "cpu", s.nbf, df_ortho_rtol=1e-10).numpy()
     Xe = ootf.Metric(M, 1e-10).Mm12
     assert np.abs(Be - np.einsum("PQ,Qmn->Pmn", Xe, T)).max() < 1e-10
+
+
+def test_linearly_dependent_metric_raises():
+    # tests/jkbuilder/test_jkbuilder.py:392-415: two identical fitting
+    # functions make (P|Q) exactly singular; the B formation and the
+    # integral-direct metric refuse it (jkbuilder.py:141-148), whatever the
+    # rounding order of the library's Cholesky leaves as the last pivot
+    import torch
+
+    from forte2_amd import integrals as ints
+    from forte2_amd.basis import BasisSet
+
+    fake = BasisSet("two identical s functions")
+    fake.add(0, [1.0], [1.0], [0.0, 0.0, 0.0])
+    fake.add(0, [1.0], [1.0], [0.0, 0.0, 0.0])
+    M = torch.from_numpy(ints.coulomb_2c(fake))
+    with pytest.raises(ValueError, match="positive definite"):
+        df.metric_factor(M)
+    with pytest.raises(ValueError, match="positive definite"):
+        ojk.build_B_density_fitting(M.numpy(), np.zeros((2, 1, 1)))
+    # a well-conditioned metric still factors
+    kind, L = df.metric_factor(M + torch.eye(2, dtype=M.dtype))
+    assert kind == "chol"

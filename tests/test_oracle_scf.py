@@ -66,6 +66,17 @@ def test_more_rhf_golden(xyz, basis, aux, unit, key, kw):
     assert r.E == pytest.approx(EREF[key]["E"], abs=5e-8)
 
 
+def test_uhf_water_cation_golden():
+    # tests/scf/test_ghf.py:24-49 (its UHF part): doublet cation, cc-pVQZ
+    g = EREF["water_cation_qz_uhf"]
+    s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit")
+    B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
+    r = oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, oscf.DFJK(B), s.Zsum - 1, 0.5,
+                     X=s.Xorth, V_sap=s.sap_potential())
+    assert r.E == pytest.approx(g["E"], abs=5e-8)
+    assert r.S2 == pytest.approx(g["S2"], abs=5e-8)
+
+
 def test_rhf_linear_dependence_golden():
     # tests/scf/test_lindep.py:9-30: H10 chain (1 bohr apart) in aug-cc-pVDZ,
     # cond(S) > 1e14, overlap_ortho_rtol 2e-7 keeps 79 of 90 orbitals

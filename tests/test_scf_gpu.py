@@ -116,6 +116,33 @@ def test_more_rhf_golden(dev, xyz, basis, aux, unit, key, kw):
     assert scf.E == pytest.approx(EREF[key]["E"], abs=5e-8)
 
 
+def test_uhf_water_cation_golden(dev):
+    # tests/scf/test_ghf.py:24-49 (its UHF part)
+    g = EREF["water_cation_qz_uhf"]
+    s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit", device=dev)
+    scf = UHF(charge=1, ms=0.5)(s).run()
+    print(f"[scf-gpu] water cation cc-pVQZ UHF: device {scf.E:.12f} <S2> {scf.S2:.10f} golden {g['E']:.12f}",
+          flush=True)
+    assert scf.E == pytest.approx(g["E"], abs=5e-8)
+    assert scf.S2 == pytest.approx(g["S2"], abs=5e-8)
+
+
+def test_linearly_dependent_metric_raises_on_the_device(dev):
+    # tests/jkbuilder/test_jkbuilder.py:392-415
+    from forte2_amd.basis import BasisSet
+    from forte2_amd.otf import FockBuilderOTF
+
+    s = System("H 0 0 0\nH 0 0 1.0", "cc-pvdz", "cc-pvtz-jkfit", unit="bohr", device=dev)
+    fake = BasisSet("two identical s functions")
+    fake.add(0, [1.0], [1.0], [0.0, 0.0, 0.0])
+    fake.add(0, [1.0], [1.0], [0.0, 0.0, 0.0])
+    s.auxiliary_basis, s.naux = fake, fake.size
+    with pytest.raises(ValueError, match="positive definite"):
+        s.fock_builder.B_Pmn
+    with pytest.raises(ValueError, match="positive definite"):
+        s.fock_builder = FockBuilderOTF(s, jk_mem_thres_mb=10, device=dev)
+
+
 def test_rhf_linear_dependence_golden(dev):
     # tests/scf/test_lindep.py:9-30: cond(S) > 1e14, 79 of 90 orbitals kept
     # (a rectangular orthogonaliser on the device), tight d_tol
