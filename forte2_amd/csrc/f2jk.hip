@@ -73,6 +73,7 @@ struct f2jk_ctx {
     int ks_tile = 0;                // BF16x3 contraction tile edge: 128, 256, 0 = by nbf
     int kx_rows = 0;                // BF16x3 half-transform tile rows: 128, 256, 0 = by nbf
     int ks_dma = -1;                // BF16x3 contraction tiles by direct-to-LDS loads: 0, 1, -1 = with 256 tiles
+    int kx_wide = -1;               // FP64/FP32 half-transform in 320-column tiles: 0, 1, -1 = above 128 columns
     hipStream_t side = nullptr;     // default priority, created with the context
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int target_blocks = kTargetBlocks;
@@ -106,6 +107,7 @@ struct Plan {
     int Pc = 0, nchunks = 0, ntile = 0, nlower = 0, nsplit = 0, ncolt = 0, kva = 1, nbw = 0, kx_waves = 4;
     int ks_tile = 128;  // edge of the contraction's lower tiles (256: BF16x3 8-wave tiles)
     int kx_rows = 128;  // rows of a BF16x3 half-transform tile (256: the tall kernel)
+    bool kx_wide = false;  // FP64/FP32 half-transform in 320-column tiles (kx_wide_kernel)
     int ctrows = 0;     // rows of the padded bf16 coefficient planes (>= ncolt * tile columns)
     // workspace offsets
     int KP = 0;                // nbf rounded up to the bf16 k step
@@ -334,7 +336,10 @@ int make_plan(const f2jk_ctx* ctx, int ndens, const int64_t* nocc, int precision
             p.kx_waves = ctx->kx_waves;
             p.ncolt = (int)cdiv(p.OPAD, 32 * p.nbw);
         } else {
-            p.ncolt = (int)cdiv(p.OPAD, kTile);
+            // one 320-column tile reads each row block of B once (the
+            // 128-column kernel once per column tile)
+            p.kx_wide = !grouped && (ctx->kx_wide < 0 ? p.OPAD > kTile : ctx->kx_wide == 1);
+            p.ncolt = (int)cdiv(p.OPAD, p.kx_wide ? kWideCols : kTile);
         }
         if (p.tk == 8)
             p.kva = (N % 2 == 0) ? 2 : 1;
@@ -517,6 +522,33 @@ int build_d(f2jk_ctx* ctx, const Plan& p, char* ws, hipStream_t st) {
 }
 
 // --------------------------------------------------------------------------- K
+// X = A Cp over `grid` tiles: 128 x 320 tiles (kx_wide_kernel) or 128 x 128
+// (kx_kernel), as the plan's column tiling says
+template <typename T>
+void launch_kx_typed(const Plan& p, unsigned grid, const T* A, int64_t R, const T* cp, T* X, hipStream_t st) {
+    constexpr int V4 = sizeof(T) == 4 ? 4 : 2;
+    if (p.kx_wide) {
+        if (p.kva == 4)
+            hipLaunchKernelGGL((kx_wide_kernel<T, V4>), dim3(grid), dim3(kWideThreads), 0, st, A, R, p.N, cp,
+                               p.OPAD, X, p.ncolt);
+        else if (p.kva == 2)
+            hipLaunchKernelGGL((kx_wide_kernel<T, 2>), dim3(grid), dim3(kWideThreads), 0, st, A, R, p.N, cp,
+                               p.OPAD, X, p.ncolt);
+        else
+            hipLaunchKernelGGL((kx_wide_kernel<T, 1>), dim3(grid), dim3(kWideThreads), 0, st, A, R, p.N, cp,
+                               p.OPAD, X, p.ncolt);
+    } else if (p.kva == 4) {
+        hipLaunchKernelGGL((kx_kernel<T, V4>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp, p.OPAD, X,
+                           p.ncolt);
+    } else if (p.kva == 2) {
+        hipLaunchKernelGGL((kx_kernel<T, 2>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp, p.OPAD, X,
+                           p.ncolt);
+    } else {
+        hipLaunchKernelGGL((kx_kernel<T, 1>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp, p.OPAD, X,
+                           p.ncolt);
+    }
+}
+
 template <typename T>
 int run_k_typed(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStream_t st) {
     const T* B = (const T*)ctx->B[sizeof(T) == 8 ? F2JK_F64 : F2JK_F32];
@@ -531,16 +563,7 @@ int run_k_typed(f2jk_ctx* ctx, const Plan& p, double* const* K, char* ws, hipStr
         const int64_t nrowt = cdiv(R, kTile);
         TIM(F2JK_PH_KX, true);
         const T* A = B + (int64_t)P0 * NN;
-        const unsigned grid = (unsigned)(nrowt * p.ncolt);
-        if (p.kva == 4)
-            hipLaunchKernelGGL((kx_kernel<T, (sizeof(T) == 4 ? 4 : 2)>), dim3(grid), dim3(kGThreads), 0,
-                               st, A, R, p.N, cp, p.OPAD, X, p.ncolt);
-        else if (p.kva == 2)
-            hipLaunchKernelGGL((kx_kernel<T, 2>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp,
-                               p.OPAD, X, p.ncolt);
-        else
-            hipLaunchKernelGGL((kx_kernel<T, 1>), dim3(grid), dim3(kGThreads), 0, st, A, R, p.N, cp,
-                               p.OPAD, X, p.ncolt);
+        launch_kx_typed<T>(p, (unsigned)(nrowt * p.ncolt), A, R, cp, X, st);
         F2JK_LAUNCHED("kx");
         TIM(F2JK_PH_KX, false);
         TIM(F2JK_PH_KS, true);
@@ -984,6 +1007,9 @@ int f2jk_set_tuning(f2jk_ctx* ctx, const char* key, int64_t value) {
     } else if (!strcmp(key, "ks_dma")) {
         if (value < -1 || value > 1) return fail(F2JK_EINVAL, "ks_dma must be -1, 0 or 1");
         ctx->ks_dma = (int)value;
+    } else if (!strcmp(key, "kx_wide")) {
+        if (value < -1 || value > 1) return fail(F2JK_EINVAL, "kx_wide must be -1, 0 or 1");
+        ctx->kx_wide = (int)value;
     } else if (!strcmp(key, "kx_rows")) {
         if (value != 0 && value != 128 && value != 256) return fail(F2JK_EINVAL, "kx_rows must be 0, 128 or 256");
         ctx->kx_rows = (int)value;
@@ -1244,12 +1270,7 @@ int f2jk_half_transform(f2jk_ctx* ctx, const double* C, int64_t ncol, int64_t ld
         const unsigned grid = (unsigned)(cdiv(R, kTile) * p.ncolt);
         double* dst = X + (P0 - p0) * p.N * ldx;
         double* Xo = direct ? dst : Xc;
-        if (p.kva == 2)
-            hipLaunchKernelGGL((kx_kernel<double, 2>), dim3(grid), dim3(kGThreads), 0, st, B + (int64_t)P0 * NN, R,
-                               p.N, cp, p.OPAD, Xo, p.ncolt);
-        else
-            hipLaunchKernelGGL((kx_kernel<double, 1>), dim3(grid), dim3(kGThreads), 0, st, B + (int64_t)P0 * NN, R,
-                               p.N, cp, p.OPAD, Xo, p.ncolt);
+        launch_kx_typed<double>(p, grid, B + (int64_t)P0 * NN, R, cp, Xo, st);
         F2JK_LAUNCHED("kx");
         if (!direct) {
             hipLaunchKernelGGL(x_copy_kernel, dim3((unsigned)cdiv(R * ncol, 256)), dim3(256), 0, st, Xc, R,

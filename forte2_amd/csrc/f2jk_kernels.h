@@ -569,6 +569,143 @@ __global__ __launch_bounds__(kGThreads) void kx_kernel(const T* __restrict__ A, 
     kx_kernel_body<T, VA>(A, R, K, Cp, OPAD, X, ncolt, blockIdx.x, gridDim.x);
 }
 
+// ---------------------------------------------------------------------------
+// KX, wide tiles (FP64 / FP32): the product of kx_kernel with 128 x 320
+// output tiles, so that a 128-row block of B is read once for up to 320
+// occupied columns (the 128-column kernel reads it once per column tile: 3x
+// at 320 columns).  8 waves as 2 x 4, each 64 x 80 (4 x 5 MFMA blocks, 160
+// accumulator registers in FP64), one workgroup per CU.  The k loop, the
+// MFMA and each output element's accumulation order are those of kx_kernel,
+// so X is bitwise the same.  The k-major coefficient tile is padded by 16
+// elements per row: its fragment reads (4 k rows x 16 columns per wave
+// instruction) then fall in distinct LDS banks.
+// ---------------------------------------------------------------------------
+constexpr int kWideCols = 320;
+constexpr int kWideThreads = 512;
+
+template <typename T, int VA>
+__device__ __forceinline__ void kx_wide_body(const T* __restrict__ A, int64_t R, int K,
+                                             const T* __restrict__ Cp, int OPAD, T* __restrict__ X,
+                                             int ncolt, int b, int nblocks) {
+    typedef typename Acc<T>::type acc_t;
+    constexpr int TN = kWideCols, BS = TN + 16;
+    constexpr int WC = 4, NB = TN / WC / 16;       // 5 column blocks per wave
+    constexpr int VB = 2;                          // coefficient loads: 2 elements each
+    constexpr int A_LD = (kTile * kBK) / (kWideThreads * VA);
+    constexpr int B_LD = (kBK * TN) / (kWideThreads * VB);
+    static_assert(A_LD * kWideThreads * VA == kTile * kBK && B_LD * kWideThreads * VB == kBK * TN,
+                  "tiles must split evenly over the threads");
+    __shared__ T As[2][kTile][kBK + 1];
+    __shared__ T Bs[2][kBK][BS];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = xcd_remap(b, nblocks);
+    const int64_t r0 = (int64_t)(t / ncolt) * kTile;
+    const int c0 = (t % ncolt) * TN;
+    const int wr = wid / WC, wc = wid % WC;
+    const int wcol0 = c0 + wc * 16 * NB;
+    int nbv = (OPAD - wcol0) / 16;
+    nbv = nbv < 0 ? 0 : (nbv > NB ? NB : nbv);
+
+    Vec<T, VA> ra[A_LD];
+    Vec<T, VB> rb[B_LD];
+    auto load_tiles = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) {
+            const int idx = tid + kWideThreads * s;
+            const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
+            const int64_t gr = r0 + row;
+            const int gk = k0 + kc;
+            if (gr < R && gk < K) {
+                ra[s] = load_vec<T, VA>(A + gr * K + gk);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VA; ++v) ra[s].v[v] = T(0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < B_LD; ++s) {
+            const int idx = tid + kWideThreads * s;
+            const int kr = idx / (TN / VB), cc = (idx % (TN / VB)) * VB;
+            const int gk = k0 + kr, gc = c0 + cc;
+            if (gk < K && gc < OPAD) {
+                rb[s] = load_vec<T, VB>(Cp + (int64_t)gk * OPAD + gc);
+            } else {
+#pragma unroll
+                for (int v = 0; v < VB; ++v) rb[s].v[v] = T(0);
+            }
+        }
+    };
+    auto store_tiles = [&](int stg) {
+#pragma unroll
+        for (int s = 0; s < A_LD; ++s) {
+            const int idx = tid + kWideThreads * s;
+            const int row = idx / (kBK / VA), kc = (idx % (kBK / VA)) * VA;
+#pragma unroll
+            for (int v = 0; v < VA; ++v) As[stg][row][kc + v] = ra[s].v[v];
+        }
+#pragma unroll
+        for (int s = 0; s < B_LD; ++s) {
+            const int idx = tid + kWideThreads * s;
+            const int kr = idx / (TN / VB), cc = (idx % (TN / VB)) * VB;
+#pragma unroll
+            for (int v = 0; v < VB; ++v) Bs[stg][kr][cc + v] = rb[s].v[v];
+        }
+    };
+
+    acc_t acc[4][NB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    const int nk = (K + kBK - 1) / kBK;
+    load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+    if (nk > 1) load_tiles(kBK);
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 4) {
+            T a[4], bv[NB];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[cur][wr * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) bv[j] = Bs[cur][kk + (lane >> 4)][wc * 16 * NB + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (j < nbv) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i][j] = mfma4(a[i], bv[j], acc[i][j]);
+                }
+            }
+        }
+        if (kt + 1 < nk) store_tiles(cur ^ 1);  // the other stage was last read before the previous barrier
+        if (kt + 2 < nk) load_tiles((kt + 2) * kBK);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j >= nbv) continue;
+            const int col = wcol0 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + wr * 64 + i * 16 + acc_row<T>(lane, r);
+                if (row < R) X[row * OPAD + col] = acc[i][j][r];
+            }
+        }
+}
+template <typename T, int VA>
+__global__ __launch_bounds__(kWideThreads) void kx_wide_kernel(const T* __restrict__ A, int64_t R, int K,
+                                                               const T* __restrict__ Cp, int OPAD,
+                                                               T* __restrict__ X, int ncolt) {
+    kx_wide_body<T, VA>(A, R, K, Cp, OPAD, X, ncolt, blockIdx.x, gridDim.x);
+}
+
 
 // ---------------------------------------------------------------------------
 // KS:  Kpart[s][m][n] (+)= sum_{P in split s} sum_{i < width} X[P][m][ioff+i] X[P][n][ioff+i]

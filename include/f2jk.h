@@ -101,7 +101,10 @@ int f2jk_get_shape(const f2jk_ctx* ctx, int64_t* naux, int64_t* nbf);
  * of an even number of 32-column units); "ks_tile" (0, 128, 256; default 0 =
  * 256 when nbf >= 768) is the lower-tile edge of the single-job BF16x3
  * contraction (256: one 8-wave workgroup per CU; grouped builds use 128);
- * "kx_rows" (0, 128, 256; default 0 = 256 when nbf >= 768 and nbf % 4 == 0)
+ * "kx_wide" (-1, 0, 1; default -1 = above 128 occupied columns) runs the
+ * FP64/FP32 half-transform in 128 x 320 tiles (8 waves), which read each
+ * row block of B once instead of once per 128-column tile (the same products,
+ * bitwise); "kx_rows" (0, 128, 256; default 0 = 256 when nbf >= 768 and nbf % 4 == 0)
  * is the row count of a single-job BF16x3 half-transform tile (256: 8 waves,
  * even widths, coefficient tiles loaded straight into LDS); "ks_dma" (-1, 0,
  * 1; default -1 = with 256-wide tiles) loads the BF16x3 contraction's X

@@ -181,6 +181,40 @@ def test_bf16x3_tall_half_transform(dev, Q, N, noccs, xchunk):
 @pytest.mark.parametrize(
     "Q,N,noccs",
     [
+        (600, 200, [150]),          # one wide tile, 160 of 320 columns occupied
+        (300, 132, [100, 90]),      # two densities side by side (208 columns)
+        (300, 240, [200, 150]),     # two wide column tiles (368 columns), the second partial
+        (200, 197, [160]),          # odd nbf (one element per A load), a k tail
+    ],
+)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fp64_fp32_wide_half_transform(dev, Q, N, noccs, prec):
+    # The 320-column half-transform (default above 128 occupied columns) runs
+    # the MFMA sequence of the 128-column kernel per output element (k steps
+    # in order): K bitwise the same, and FP64 within 1e-10 of the oracle.
+    B, Cs, Bh, Ch = _case(dev, Q, N, noccs, seed=31)
+    Jr, Kr = ojk.build_JK(Bh, Ch)
+    out = {}
+    for wide in (0, 1):
+        fb = FockBuilder(B=B, device=dev)
+        fb.set_tuning("kx_wide", wide)
+        J, K = fb.build_JK(Cs, precision=prec)
+        out[wide] = K
+        for k in range(len(noccs)):
+            dk = np.abs(K[k].cpu().numpy() - Kr[k]).max()
+            print(f"   {prec} kx_wide={wide} dens {k}: |dK|={dk:.2e}", flush=True)
+            if prec == "fp64":
+                assert dk <= 1e-10
+            else:
+                assert dk <= 1e-5 * np.abs(Kr[k]).max()
+        fb.release()
+    for k in range(len(noccs)):
+        assert torch.equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize(
+    "Q,N,noccs",
+    [
         (2224, 384, [80]),        # (H2O)_16
         (300, 130, [17, 13]),     # two densities, rows past nbf clamped in the last tile
         (100, 19, [5]),           # odd nbf, one partial tile
