@@ -252,6 +252,10 @@ def test_error_paths(dev):
         fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)] * 9)
     with pytest.raises(ValueError):
         fb.build_JK([torch.zeros((8, 2), dtype=torch.float64, device=dev)], precision="fp16")
+    # launch-shape knobs outside their documented values are refused
+    for key, value in (("kx_wide", 2), ("kx_rows", 64), ("ks_tile", 96), ("ks_dma", -2), ("no_such_knob", 1)):
+        with pytest.raises(ValueError):
+            fb.set_tuning(key, value)
 
 
 def test_edge_shapes(dev):
