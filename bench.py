@@ -682,7 +682,11 @@ def main():
     # unless the launch shape says 128 (the library's rule: 256 for nbf >= 768)
     rows = fb.knobs.get("kx_rows", 0) or (256 if N >= 768 and N % 4 == 0 else 128)
     kx_name = "kx_bf16x3_tall_kernel" if rows == 256 else "kx_bf16x3_kernel"
-    kx_prefix = f"f2jk::{kx_name}" if args.mixed == "bf16x3" else "f2jk::kx_kernel<float"
+    if args.mixed != "bf16x3":
+        # FP32: 320-column tiles above 128 occupied columns (the library's rule)
+        wide = fb.knobs.get("kx_wide", -1)
+        kx_name = "kx_wide_kernel" if (wide == 1 or (wide < 0 and o > 128)) else "kx_kernel"
+    kx_prefix = f"f2jk::{kx_name}" if args.mixed == "bf16x3" else f"f2jk::{kx_name}<float"
     traffic, traffic_note = measured_traffic(args.waters, args.mixed, kx_prefix)
 
     # the synthetic B leaves HBM before the real-integral legs
@@ -742,7 +746,7 @@ def main():
         },
         "roofline": {
             "kernel": f"{kx_name} (K half-transform X = B C, {rows}-row tiles)" if args.mixed == "bf16x3"
-                      else "kx_kernel (K half-transform X = B C)",
+                      else f"{kx_name} (K half-transform X = B C)",
             "bound": "mfma",
             "achieved": achieved,
             "peak": peak,
