@@ -867,24 +867,30 @@ void eval_fam4(const f2ints_basis* bs, const FamPair& f, const Shell* const* Ps,
                 }
             }
         }
-        // V4[(sP * nab + s2) * kLanes + lane], h ascending as in eval_fam
+        // V4[lane * nsP * nab + sP * nab + s2], h ascending as in eval_fam; the
+        // kLanes sums of one s2 stay in registers over the h loop, and each
+        // lane's values are contiguous for the column accumulation below
+        const int nv = nsP * nab;
         for (int sP = 0; sP < nsP; ++sP) {
             const double* y = &w.Y4[(size_t)sP * nhab * kLanes];
-            double* v = &w.V4[(size_t)sP * nab * kLanes];
-            std::fill(v, v + (size_t)nab * kLanes, 0.0);
-            for (int h = 0; h < nhab; ++h) {
-                const double* e = &pp.E[(size_t)h * nab];
-                const double* yh = y + (size_t)h * kLanes;
-                for (int s2 = 0; s2 < nab; ++s2)
-                    for (int l = 0; l < kLanes; ++l) v[s2 * kLanes + l] = std::fma(e[s2], yh[l], v[s2 * kLanes + l]);
+            for (int s2 = 0; s2 < nab; ++s2) {
+                double acc[kLanes] = {};
+                const double* e = &pp.E[s2];
+                for (int h = 0; h < nhab; ++h) {
+                    const double eh = e[(size_t)h * nab];
+                    const double* yh = y + (size_t)h * kLanes;
+                    for (int l = 0; l < kLanes; ++l) acc[l] = std::fma(eh, yh[l], acc[l]);
+                }
+                for (int l = 0; l < kLanes; ++l) w.V4[(size_t)l * nv + sP * nab + s2] = acc[l];
             }
         }
         for (int c = 0; c < ncp; ++c) {
             if (pp.w[c] == 0.0) continue;
             for (int l = 0; l < kLanes; ++l) {
                 const double pref = base[l] * pp.w[c];
-                double* a = &w.acc4[l][(size_t)c * nsP * nab];
-                for (int i = 0; i < nsP * nab; ++i) a[i] = std::fma(pref, w.V4[(size_t)i * kLanes + l], a[i]);
+                double* a = &w.acc4[l][(size_t)c * nv];
+                const double* v = &w.V4[(size_t)l * nv];
+                for (int i = 0; i < nv; ++i) a[i] = std::fma(pref, v[i], a[i]);
             }
         }
     }
