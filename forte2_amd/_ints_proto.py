@@ -15,6 +15,7 @@ F2INTS_SYMBOLS = (
     "f2ints_coulomb_3c",
     "f2ints_ket_pairs",
     "f2ints_coulomb_3c_pairs",
+    "f2ints_coulomb_3c_pairs_t",
     "f2ints_coulomb_3c_pairs_aux",
     "f2ints_coulomb_3c_sum",
     "f2ints_coulomb_4c",
@@ -43,6 +44,7 @@ def declare(lib):
     lib.f2ints_ket_pairs.argtypes = [V, Ip, Ip, I64]
     lib.f2ints_ket_pairs.restype = I64
     lib.f2ints_coulomb_3c_pairs.argtypes = [V, V, I64, I64, Dp, I64]
+    lib.f2ints_coulomb_3c_pairs_t.argtypes = [V, V, I64, I64, Dp, I64]
     lib.f2ints_coulomb_3c_pairs_aux.argtypes = [V, V, I, I, I64, I64, Dp, I64]
     lib.f2ints_coulomb_3c_sum.argtypes = [V, V, Dp]
     lib.f2ints_coulomb_4c.argtypes = [V, Dp]

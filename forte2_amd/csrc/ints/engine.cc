@@ -1027,16 +1027,20 @@ int64_t pair_ncols(const f2ints_basis* bs, const std::pair<int, int>& ab) {
 // m >= n appears once.  The family pairs that hold these pairs are evaluated
 // whole; their other column pairs are not written.
 // Rows P of the aux shells [s0, s1) (row 0 = the first function of shell s0),
-// packed columns of the pairs [q0, q1).
+// packed columns of the pairs [q0, q1).  With `colmajor` the same values go
+// to out[c * ldo + P]: a thread then fills whole rows of its columns instead
+// of a narrow strip of every row.
 int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, int s1, int64_t q0, int64_t q1,
-                     double* out, int64_t ldo) {
+                     double* out, int64_t ldo, bool colmajor = false) {
     const std::vector<std::pair<int, int>>& pairs = ket_pairs(bs);
     if (q0 < 0 || q1 > (int64_t)pairs.size() || q0 > q1) return fail("f2ints_coulomb_3c_pairs: bad pair range");
     if (s0 < 0 || s1 > (int)aux->shells.size() || s0 >= s1) return fail("f2ints_coulomb_3c_pairs: bad aux shell range");
     const int64_t prow0 = aux->shells[s0].off;
     std::vector<int64_t> col(q1 - q0 + 1, 0);
     for (int64_t q = q0; q < q1; ++q) col[q - q0 + 1] = col[q - q0] + pair_ncols(bs, pairs[q]);
-    if (ldo < col.back()) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the block's columns");
+    const int64_t nrows = aux->shells[s1 - 1].off + nsph(aux->shells[s1 - 1].l) - prow0;
+    if (!colmajor && ldo < col.back()) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the block's columns");
+    if (colmajor && ldo < nrows) return fail("f2ints_coulomb_3c_pairs: ldo smaller than the aux functions");
     std::vector<int> fams;
     for (int64_t q = q0; q < q1; ++q) fams.push_back(bs->fam_of_pair[q]);
     std::sort(fams.begin(), fams.end());
@@ -1056,6 +1060,17 @@ int coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int s0, in
                     const int64_t q = f.q[c];
                     if (q < q0 || q >= q1) continue;
                     const bool diag = pairs[q].first == pairs[q].second;
+                    if (colmajor) {
+                        // column t of the pair: rows P.off - prow0 + sP, sP ascending
+                        double* o = out + (size_t)col[q - q0] * ldo + (P.off - prow0);
+                        int t = 0;
+                        for (int sa = 0; sa < na; ++sa)
+                            for (int sb = 0; sb < (diag ? sa + 1 : nb); ++sb, ++t) {
+                                double* dst = o + (size_t)t * ldo;
+                                for (int sP = 0; sP < nsP; ++sP) dst[sP] = acc[(c * nsP + sP) * nab + sa * nb + sb];
+                            }
+                        continue;
+                    }
                     double* o = out + col[q - q0];
                     for (int sP = 0; sP < nsP; ++sP) {
                         double* row = o + (size_t)(P.off - prow0 + sP) * ldo;
@@ -1262,6 +1277,13 @@ int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int
     if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_pairs: null argument");
     if (q0 == q1 || aux->shells.empty()) return 0;
     return coulomb_3c_pairs(aux, bs, 0, (int)aux->shells.size(), q0, q1, out, ldo);
+}
+
+int f2ints_coulomb_3c_pairs_t(const f2ints_basis* aux, const f2ints_basis* bs, int64_t q0, int64_t q1, double* out,
+                              int64_t ldo) {
+    if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_pairs_t: null argument");
+    if (q0 == q1 || aux->shells.empty()) return 0;
+    return coulomb_3c_pairs(aux, bs, 0, (int)aux->shells.size(), q0, q1, out, ldo, true);
 }
 
 int f2ints_coulomb_3c_pairs_aux(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin, int shell_end,

@@ -127,12 +127,14 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
     host_s = [0.0]
 
     def packed(flat, w):
-        return flat[: Q * w].view(Q, w)  # contiguous (Q, w) block of a flat buffer
+        # contiguous (w, Q) block of a flat buffer: one row of naux values per
+        # packed column, the engine's column-major output
+        return flat[: Q * w].view(w, Q)
 
     def evaluate(k, slot):
         q0, q1, c0, c1 = blocks[k]
         t0 = time.perf_counter()
-        _ints.coulomb_3c_pairs(aux, bas, q0, q1, packed(host[slot], c1 - c0).numpy())
+        _ints.coulomb_3c_pairs_t(aux, bas, q0, q1, packed(host[slot], c1 - c0).numpy())
         host_s[0] += time.perf_counter() - t0
 
     aux.handle, bas.handle  # engine handles made here, before two threads use them
@@ -160,13 +162,13 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
                 if copied[nxt] is not None:
                     copied[nxt].synchronize()
                 fut = pool.submit(evaluate, k + 1, nxt)
-            T = packed(dev_in, c1 - c0)
-            T.copy_(packed(host[slot], c1 - c0), non_blocking=pin)
+            Tt = packed(dev_in, c1 - c0)
+            Tt.copy_(packed(host[slot], c1 - c0), non_blocking=pin)
             if pin:
                 ev = torch.cuda.Event()
                 ev.record(stream)
                 copied[slot] = ev
-            Bp = X @ T
+            Bp = X @ Tt.T
             Bv.index_copy_(1, idx_nm[c0:c1], Bp)
             Bv.index_copy_(1, idx_mn[c0:c1], Bp)
             del Bp
@@ -183,36 +185,37 @@ def build_B_density_fitting(system, device, block_bytes=1 << 30, progress=None, 
 
 def host_df_integrals(system):
     """Host part of B formation for a job prepared off the serving thread:
-    (M, packed) with M = (P|Q) and packed = (T, m, n), T the unique half of
-    (Q|mn) over the significant shell pairs as a (naux, ncols) array and m, n
-    the basis functions of its columns (pair_columns)."""
+    (M, packed) with M = (P|Q) and packed = (Tt, m, n), Tt the unique half of
+    (Q|mn) over the significant shell pairs as an (ncols, naux) array (one row
+    per packed column) and m, n the basis functions of its columns
+    (pair_columns)."""
     import numpy as np
 
     aux, bas = system.auxiliary_basis, system.basis
     col0, m, n = pair_columns(bas)
-    T = np.empty((aux.size, int(col0[-1])))
-    if T.shape[1]:
-        _ints.coulomb_3c_pairs(aux, bas, 0, len(col0) - 1, T)
-    return _ints.coulomb_2c(aux), (T, m, n)
+    Tt = np.empty((int(col0[-1]), aux.size))
+    if Tt.shape[0]:
+        _ints.coulomb_3c_pairs_t(aux, bas, 0, len(col0) - 1, Tt)
+    return _ints.coulomb_2c(aux), (Tt, m, n)
 
 
 def device_B_from_host(M, packed, device, nbf, df_ortho_rtol=None, block_bytes=1 << 30):
     """B = X T on the device from the host_df_integrals output (same algebra
     as build_B_density_fitting): the packed columns are multiplied by X in
     blocks and scattered to (m, n) and (n, m)."""
-    T, m, n = packed
-    Q, N = T.shape[0], int(nbf)
+    Tt, m, n = packed
+    Q, N = Tt.shape[1], int(nbf)
     kind, fac = metric_factor(torch.from_numpy(M).to(device), df_ortho_rtol)
     X = _fitting_matrix(kind, fac)
     B = torch.zeros((Q, N, N), dtype=torch.float64, device=device)
     Bv = B.view(Q, N * N)
     idx_mn = torch.from_numpy(m * N + n).to(device)
     idx_nm = torch.from_numpy(n * N + m).to(device)
-    ncols = T.shape[1]
+    ncols = Tt.shape[0]
     width = max(1, block_bytes // (8 * Q))
     for c0 in range(0, ncols, width):
         c1 = min(ncols, c0 + width)
-        Bp = X @ torch.from_numpy(np.ascontiguousarray(T[:, c0:c1])).to(device)
+        Bp = X @ torch.from_numpy(Tt[c0:c1]).to(device).T
         Bv.index_copy_(1, idx_nm[c0:c1], Bp)
         Bv.index_copy_(1, idx_mn[c0:c1], Bp)
     return B

@@ -100,6 +100,15 @@ def coulomb_3c_pairs(aux, basis, q0, q1, out):
     return out
 
 
+def coulomb_3c_pairs_t(aux, basis, q0, q1, out):
+    """coulomb_3c_pairs stored column-major: out[c, P] (out: (>= ncols, naux)
+    C-contiguous float64), each packed column a contiguous row."""
+    if out.ndim != 2 or out.shape[1] != aux.size or not out.flags.c_contiguous or out.dtype != np.float64:
+        raise ValueError("out has the wrong shape/layout")
+    _check(_lib().f2ints_coulomb_3c_pairs_t(aux.handle, basis.handle, int(q0), int(q1), _ptr(out), out.shape[1]))
+    return out
+
+
 def coulomb_3c_pairs_aux(aux, basis, s0, s1, q0, q1, out):
     """coulomb_3c_pairs for the aux shells [s0, s1) only: out (rows = the
     functions of those shells, ldo) C-contiguous float64."""

@@ -75,6 +75,12 @@ int64_t f2ints_ket_pairs(const f2ints_basis* bs, int* a, int* b, int64_t cap);
 int f2ints_coulomb_3c_pairs(const f2ints_basis* aux, const f2ints_basis* bs, int64_t pair_begin,
                             int64_t pair_end, double* out, int64_t ldo);
 
+/* The same packed (P|ab) stored column-major, out[c * ldo + P] (ldo >= naux):
+ * each packed column (one (m, n), m >= n) is a contiguous row of naux values.
+ * The device B formation multiplies these rows by L^-1 as they are. */
+int f2ints_coulomb_3c_pairs_t(const f2ints_basis* aux, const f2ints_basis* bs, int64_t pair_begin,
+                              int64_t pair_end, double* out, int64_t ldo);
+
 /* The same packed (P|ab) for the aux shells [shell_begin, shell_end) only:
  * row 0 of out is the first function of shell_begin.  Used by the
  * integral-direct builder for its aux-shell blocks of (P|mn)

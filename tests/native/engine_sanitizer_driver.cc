@@ -89,6 +89,17 @@ int main() {
     for (int64_t P = 0; P < Q; ++P)
         for (int64_t c = 0; c < nc; ++c)
             if (packed[P * nc + c] != sq[(P * N + colm[c]) * N + coln[c]]) return fail("packed != square");
+    // the column-major layout holds the same values, one row per column
+    // (row stride Q + 3: the padding must stay untouched)
+    std::vector<double> packed_t(nc * (Q + 3), -7.0);
+    if (f2ints_coulomb_3c_pairs_t(ax, bs, 0, npairs, packed_t.data(), Q + 3)) return fail("coulomb_3c_pairs_t");
+    for (int64_t c = 0; c < nc; ++c) {
+        for (int64_t P = 0; P < Q; ++P)
+            if (packed_t[c * (Q + 3) + P] != packed[P * nc + c]) return fail("column-major != row-major");
+        for (int64_t P = Q; P < Q + 3; ++P)
+            if (packed_t[c * (Q + 3) + P] != -7.0) return fail("column-major wrote past naux");
+    }
+    if (f2ints_coulomb_3c_pairs_t(ax, bs, 0, npairs, packed_t.data(), Q - 1) == 0) return fail("short ldo accepted");
     // aux-shell ranges that cut the runs of one l in different places
     std::vector<int> aoff(nsh_aux + 1, 0);
     for (int s = 0; s < nsh_aux; ++s) aoff[s + 1] = aoff[s] + 2 * aux.l[s] + 1;

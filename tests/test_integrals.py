@@ -161,6 +161,14 @@ def test_three_center_pair_blocks_match_square():
         assert np.isnan(out[:, c1 - c0:]).all()  # nothing past the block's columns
     with pytest.raises(RuntimeError):
         ints.coulomb_3c_pairs(aux, b, 0, npairs, np.empty((aux.size, 4)))
+    # the column-major layout (f2ints_coulomb_3c_pairs_t): the same values,
+    # one row per packed column, over the same pair ranges
+    for q0, q1 in ((0, npairs), (5, 17)):
+        c0, c1 = col0[q0], col0[q1]
+        out_t = np.full((c1 - c0 + 2, aux.size), np.nan)
+        ints.coulomb_3c_pairs_t(aux, b, q0, q1, out_t)
+        assert np.array_equal(out_t[: c1 - c0].T, full[:, m[c0:c1], n[c0:c1]])
+        assert np.isnan(out_t[c1 - c0:]).all()
 
 
 def test_three_center_pairs_by_aux_shells():
