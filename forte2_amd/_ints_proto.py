@@ -16,6 +16,7 @@ F2INTS_SYMBOLS = (
     "f2ints_ket_pairs",
     "f2ints_coulomb_3c_pairs",
     "f2ints_coulomb_3c_pairs_t",
+    "f2ints_coulomb_3c_pairs_aux_t",
     "f2ints_coulomb_3c_pairs_aux",
     "f2ints_coulomb_3c_sum",
     "f2ints_coulomb_4c",
@@ -46,6 +47,7 @@ def declare(lib):
     lib.f2ints_coulomb_3c_pairs.argtypes = [V, V, I64, I64, Dp, I64]
     lib.f2ints_coulomb_3c_pairs_t.argtypes = [V, V, I64, I64, Dp, I64]
     lib.f2ints_coulomb_3c_pairs_aux.argtypes = [V, V, I, I, I64, I64, Dp, I64]
+    lib.f2ints_coulomb_3c_pairs_aux_t.argtypes = [V, V, I, I, I64, I64, Dp, I64]
     lib.f2ints_coulomb_3c_sum.argtypes = [V, V, Dp]
     lib.f2ints_coulomb_4c.argtypes = [V, Dp]
     lib.f2ints_boys.argtypes = [I, D, Dp]

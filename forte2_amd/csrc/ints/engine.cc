@@ -1293,6 +1293,13 @@ int f2ints_coulomb_3c_pairs_aux(const f2ints_basis* aux, const f2ints_basis* bs,
     return coulomb_3c_pairs(aux, bs, shell_begin, shell_end, q0, q1, out, ldo);
 }
 
+int f2ints_coulomb_3c_pairs_aux_t(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin, int shell_end,
+                                  int64_t q0, int64_t q1, double* out, int64_t ldo) {
+    if (!aux || !bs || !out) return fail("f2ints_coulomb_3c_pairs_aux_t: null argument");
+    if (q0 == q1 || shell_begin == shell_end) return 0;
+    return coulomb_3c_pairs(aux, bs, shell_begin, shell_end, q0, q1, out, ldo, true);
+}
+
 int f2ints_coulomb_4c(const f2ints_basis* b, double* out) {
     if (!b || !out) return fail("f2ints_coulomb_4c: null argument");
     if (b->nbf > 200) return fail("f2ints_coulomb_4c: basis too large for the test-only 4-centre routine");

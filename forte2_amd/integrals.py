@@ -91,6 +91,17 @@ def ket_pairs(basis):
     return a, b
 
 
+def packed_columns(basis, q0, q1):
+    """Packed columns of the significant pairs [q0, q1): na*nb for a > b,
+    na(na+1)/2 for a == b (the layout of coulomb_3c_pairs)."""
+    a, b = ket_pairs(basis)
+    if not (0 <= q0 <= q1 <= len(a)):
+        raise ValueError("bad pair range")
+    ls = np.array([l for l, *_ in basis.shells], dtype=np.int64)
+    na, nb = 2 * ls[a[q0:q1]] + 1, 2 * ls[b[q0:q1]] + 1
+    return int(np.where(a[q0:q1] == b[q0:q1], na * (na + 1) // 2, na * nb).sum())
+
+
 def coulomb_3c_pairs(aux, basis, q0, q1, out):
     """(P|ab) of all aux functions and the pairs [q0, q1) of ket_pairs,
     packed into out[:, :ncols] (out: (naux, ldo) C-contiguous float64)."""
@@ -105,6 +116,8 @@ def coulomb_3c_pairs_t(aux, basis, q0, q1, out):
     C-contiguous float64), each packed column a contiguous row."""
     if out.ndim != 2 or out.shape[1] != aux.size or not out.flags.c_contiguous or out.dtype != np.float64:
         raise ValueError("out has the wrong shape/layout")
+    if out.shape[0] < packed_columns(basis, q0, q1):
+        raise ValueError("out has fewer rows than the pairs have packed columns")
     _check(_lib().f2ints_coulomb_3c_pairs_t(aux.handle, basis.handle, int(q0), int(q1), _ptr(out), out.shape[1]))
     return out
 
@@ -119,6 +132,21 @@ def coulomb_3c_pairs_aux(aux, basis, s0, s1, q0, q1, out):
         raise ValueError("out has the wrong shape/layout")
     _check(_lib().f2ints_coulomb_3c_pairs_aux(aux.handle, basis.handle, int(s0), int(s1), int(q0), int(q1),
                                                _ptr(out), out.shape[1]))
+    return out
+
+
+def coulomb_3c_pairs_aux_t(aux, basis, s0, s1, q0, q1, out):
+    """coulomb_3c_pairs_aux stored column-major: out[c, P - first] (out:
+    (>= ncols, ldo >= the block's rows) C-contiguous float64)."""
+    offs = aux.shell_offsets()
+    if not (0 <= s0 < s1 <= len(offs) - 1):
+        raise ValueError("bad aux shell range")
+    if out.ndim != 2 or out.shape[1] < offs[s1] - offs[s0] or not out.flags.c_contiguous or out.dtype != np.float64:
+        raise ValueError("out has the wrong shape/layout")
+    if out.shape[0] < packed_columns(basis, q0, q1):
+        raise ValueError("out has fewer rows than the pairs have packed columns")
+    _check(_lib().f2ints_coulomb_3c_pairs_aux_t(aux.handle, basis.handle, int(s0), int(s1), int(q0), int(q1),
+                                                 _ptr(out), out.shape[1]))
     return out
 
 

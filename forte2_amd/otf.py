@@ -137,10 +137,12 @@ class FockBuilderOTF:
         self._idx_mn = torch.from_numpy(mcol * N + ncol).to(self.device)
         self._idx_nm = torch.from_numpy(ncol * N + mcol).to(self.device)
         self._dev_block = torch.zeros((self.pblksize, N, N), dtype=torch.float64, device=self.device)
+        # staging of a block: one row of pblksize values per packed column
+        # (the engine's column-major output: each thread fills whole rows)
         w = max(1, self._ncols)
-        self._dev_packed = torch.empty((self.pblksize, w), dtype=torch.float64, device=self.device)
+        self._dev_packed = torch.empty((w, self.pblksize), dtype=torch.float64, device=self.device)
         pin = self.device.type == "cuda"
-        self._host = [torch.empty((self.pblksize, w), dtype=torch.float64, pin_memory=pin) for _ in range(2)]
+        self._host = [torch.empty((w, self.pblksize), dtype=torch.float64, pin_memory=pin) for _ in range(2)]
         self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         # a device FockBuilder whose B is re-pointed at each block
         self._kx = FockBuilder(B=self._dev_block, device=self.device)
@@ -183,8 +185,8 @@ class FockBuilderOTF:
         s0, s1, p0, p1 = self.blocks[k]
         t0 = time.perf_counter()
         if self._ncols:
-            _ints.coulomb_3c_pairs_aux(self.auxbasis, self.basis, s0, s1, 0, self._npairs,
-                                       self._host[slot][: p1 - p0].numpy())
+            _ints.coulomb_3c_pairs_aux_t(self.auxbasis, self.basis, s0, s1, 0, self._npairs,
+                                         self._host[slot].numpy())
         self.host_seconds += time.perf_counter() - t0
 
     def _sweep(self):
@@ -224,8 +226,8 @@ class FockBuilderOTF:
                     self._cache[k] = T
                 else:
                     T = self._dev_block[:nP]
-                packed = self._dev_packed[:nP, : self._ncols]
-                packed.copy_(self._host[slot][:nP, : self._ncols], non_blocking=True)
+                self._dev_packed.copy_(self._host[slot], non_blocking=True)
+                packed = self._dev_packed[: self._ncols, :nP].T
                 ev = torch.cuda.Event()
                 ev.record(stream)
                 copied[slot] = ev

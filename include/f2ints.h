@@ -88,6 +88,11 @@ int f2ints_coulomb_3c_pairs_t(const f2ints_basis* aux, const f2ints_basis* bs, i
 int f2ints_coulomb_3c_pairs_aux(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin, int shell_end,
                                 int64_t pair_begin, int64_t pair_end, double* out, int64_t ldo);
 
+/* f2ints_coulomb_3c_pairs_aux stored column-major: out[c * ldo + P - first]
+ * (ldo >= the rows of those shells), one row per packed column. */
+int f2ints_coulomb_3c_pairs_aux_t(const f2ints_basis* aux, const f2ints_basis* bs, int shell_begin,
+                                  int shell_end, int64_t pair_begin, int64_t pair_end, double* out, int64_t ldo);
+
 /* V[m][n] = sum over all functions P of `aux` of (P|mn), nbf x nbf: the
  * superposition-of-atomic-potentials guess potential of
  * forte2/scf/scf_utils.py:10-51 (aux = the SAP charge-density basis), without

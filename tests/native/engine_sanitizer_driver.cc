@@ -108,6 +108,11 @@ int main() {
         const int64_t rows = aoff[r[1]] - aoff[r[0]];
         std::vector<double> part(rows * nc);
         if (f2ints_coulomb_3c_pairs_aux(ax, bs, r[0], r[1], 0, npairs, part.data(), nc)) return fail("pairs_aux");
+        std::vector<double> part_t(nc * rows);
+        if (f2ints_coulomb_3c_pairs_aux_t(ax, bs, r[0], r[1], 0, npairs, part_t.data(), rows)) return fail("pairs_aux_t");
+        for (int64_t c = 0; c < nc; ++c)
+            for (int64_t P = 0; P < rows; ++P)
+                if (part_t[c * rows + P] != part[P * nc + c]) return fail("pairs_aux_t != pairs_aux");
         if (std::memcmp(part.data(), packed.data() + aoff[r[0]] * nc, sizeof(double) * rows * nc) != 0)
             return fail("pairs_aux != pairs");
     }

@@ -169,6 +169,9 @@ def test_three_center_pair_blocks_match_square():
         ints.coulomb_3c_pairs_t(aux, b, q0, q1, out_t)
         assert np.array_equal(out_t[: c1 - c0].T, full[:, m[c0:c1], n[c0:c1]])
         assert np.isnan(out_t[c1 - c0:]).all()
+        assert ints.packed_columns(b, q0, q1) == c1 - c0
+    with pytest.raises(ValueError):  # fewer rows than packed columns: refused before the engine runs
+        ints.coulomb_3c_pairs_t(aux, b, 0, npairs, np.empty((int(col0[-1]) - 1, aux.size)))
 
 
 def test_three_center_pairs_by_aux_shells():
@@ -189,6 +192,11 @@ def test_three_center_pairs_by_aux_shells():
         out = np.empty((offs[s1] - offs[s0], nc))
         ints.coulomb_3c_pairs_aux(aux, b, s0, s1, 0, npairs, out)
         assert np.array_equal(out, whole[offs[s0]:offs[s1]])
+        # column-major, rows padded past the block (the padding untouched)
+        out_t = np.full((nc, offs[s1] - offs[s0] + 3), np.nan)
+        ints.coulomb_3c_pairs_aux_t(aux, b, s0, s1, 0, npairs, out_t)
+        assert np.array_equal(out_t[:, : offs[s1] - offs[s0]], out.T)
+        assert np.isnan(out_t[:, offs[s1] - offs[s0]:]).all()
         sq = ints.coulomb_3c(aux, b, s0, s1)
         rebuilt = np.zeros_like(sq)
         rebuilt[:, m, n] = out
