@@ -570,12 +570,20 @@ int coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* out) {
     std::vector<std::pair<int, int>> pairs;
     for (int i = 0; i < na; ++i)
         for (int j = 0; j < (same ? i + 1 : nb); ++j) pairs.push_back({i, j});
-#pragma omp parallel for schedule(dynamic) num_threads(nthreads())
+    int lmax = 0;
+    for (const Shell& s : a->shells) lmax = std::max(lmax, s.l);
+    for (const Shell& s : b->shells) lmax = std::max(lmax, s.l);
+#pragma omp parallel num_threads(nthreads())
+    {
+    // per-thread buffers, sized once for the largest pair
+    std::vector<double> R(nherm(2 * lmax)), work((size_t)(2 * lmax + 1) * nherm(2 * lmax)),
+        acc((size_t)nsph(lmax) * nsph(lmax));
+#pragma omp for schedule(dynamic)
     for (size_t q = 0; q < pairs.size(); ++q) {
         const Shell& P = a->shells[pairs[q].first];
         const Shell& Q = b->shells[pairs[q].second];
-        const int L = P.l + Q.l, nh = nherm(L), nsP = nsph(P.l), nsQ = nsph(Q.l);
-        std::vector<double> R(nh), work((size_t)(L + 1) * nh), acc((size_t)nsP * nsQ, 0.0);
+        const int L = P.l + Q.l, nsP = nsph(P.l), nsQ = nsph(Q.l);
+        std::fill(acc.begin(), acc.begin() + (size_t)nsP * nsQ, 0.0);
         const double X = P.A[0] - Q.A[0], Y = P.A[1] - Q.A[1], Z = P.A[2] - Q.A[2];
         const double sgnQ = (Q.l & 1) ? -1.0 : 1.0;
         for (size_t ip = 0; ip < P.alpha.size(); ++ip)
@@ -604,6 +612,7 @@ int coulomb_2c(const f2ints_basis* a, const f2ints_basis* b, double* out) {
                 out[m * nbfb + n] = acc[(size_t)sP * nsQ + sQ];
                 if (same) out[n * nbfb + m] = acc[(size_t)sP * nsQ + sQ];
             }
+    }
     }
     return 0;
 }
