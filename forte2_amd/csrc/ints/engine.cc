@@ -96,6 +96,38 @@ struct BoysTable {
             for (int m = 0; m < nmax; ++m) F[m + 1] = ((2 * m + 1) * F[m] - eT) / (2.0 * T);
         }
     }
+    // eval for K arguments at once, F[lane][n]: each lane the arithmetic of
+    // eval in the same order (bit for bit its values); when every argument
+    // is on the grid the Taylor sums and the downward recursions run across
+    // the lanes
+    template <int K, int NF>
+    void eval_lanes(int nmax, const double* T, double (*F)[NF]) const {
+        for (int l = 0; l < K; ++l)
+            if (!(T[l] < Tswitch)) {
+                for (int q = 0; q < K; ++q) eval(nmax, T[q], F[q]);
+                return;
+            }
+        const double* t[K];
+        double d[K], f[K], c[K], eT[K];
+        for (int l = 0; l < K; ++l) {
+            const int i = (int)(T[l] / h + 0.5);
+            d[l] = T[l] - i * h;
+            t[l] = &tab[(size_t)i * NT];
+            f[l] = 0.0;
+            c[l] = 1.0;
+        }
+        for (int k = 0; k <= 6; ++k)
+            for (int l = 0; l < K; ++l) {
+                f[l] += t[l][nmax + k] * c[l];
+                c[l] *= -d[l] / (k + 1);
+            }
+        for (int l = 0; l < K; ++l) {
+            F[l][nmax] = f[l];
+            eT[l] = std::exp(-T[l]);
+        }
+        for (int m = nmax - 1; m >= 0; --m)
+            for (int l = 0; l < K; ++l) F[l][m] = (2.0 * T[l] * F[l][m + 1] + eT[l]) / (2 * m + 1);
+    }
 };
 const BoysTable& boys() {
     static BoysTable t;
@@ -281,8 +313,9 @@ constexpr int kLanes = 4;
 void compute_R4(int L, const double* alpha, const double* X, const double* Y, const double* Z, double* R4,
                 double* work4) {
     const int nh = nherm(L);
-    double F[kLanes][NBOYS + 2];
-    for (int l = 0; l < kLanes; ++l) boys().eval(L, alpha[l] * std::fma(Z[l], Z[l], std::fma(Y[l], Y[l], X[l] * X[l])), F[l]);
+    double F[kLanes][NBOYS + 2], T[kLanes];
+    for (int l = 0; l < kLanes; ++l) T[l] = alpha[l] * std::fma(Z[l], Z[l], std::fma(Y[l], Y[l], X[l] * X[l]));
+    boys().eval_lanes<kLanes, NBOYS + 2>(L, T, F);
     double m2a[kLanes];
     for (int l = 0; l < kLanes; ++l) m2a[l] = 1.0;
     for (int n = 0; n <= L; ++n)
