@@ -30,14 +30,16 @@ def _err(a, b):
     return float(np.abs(a - b).max())
 
 
-@pytest.mark.parametrize("Q,N,ncol", [(300, 131, 21), (139, 24, 21), (300, 131, 32)])
+@pytest.mark.parametrize("Q,N,ncol", [(300, 131, 21), (139, 24, 21), (300, 131, 32), (200, 260, 200), (120, 340, 336)])
 def test_half_transform_blocks(dev, Q, N, ncol):
     # 21 columns: a column slice (row stride 30), written through the
-    # workspace; 32 columns: X rows as wide as the kernel's, written in place
+    # workspace; 32 columns: X rows as wide as the kernel's, written in place;
+    # 200 and 336 columns: the 320-column tiles (one tile through the
+    # workspace; two column tiles, the second partial, in place)
     print(f"[mo-gpu] half-transform Q={Q} N={N} ncol={ncol}", flush=True)
     B = synthetic_B(Q, N, seed=Q, device=dev)
     Bh = B.cpu().numpy()
-    wide = orthonormal_C(N, 32, seed=3, device=dev)
+    wide = orthonormal_C(N, max(32, ncol), seed=3, device=dev)
     C = wide[:, 4:25] if ncol == 21 else wide
     fb = FockBuilder(B=B, device=dev)
     fb.set_tuning("xchunk_bytes", 1 << 20)  # several kernel chunks per block
