@@ -11,9 +11,9 @@ import json
 import os
 import sys
 
-SETS = ["sto-3g", "cc-pvdz", "cc-pvtz", "cc-pvqz", "cc-pvtz-jkfit", "cc-pvqz-jkfit",
+SETS = ["sto-3g", "sto-6g", "cc-pvdz", "cc-pvtz", "cc-pvqz", "cc-pvtz-jkfit", "cc-pvqz-jkfit",
         "def2-universal-jkfit", "sap_helfem_large", "aug-cc-pvdz"]
-ELEMENTS = ["1", "3", "6", "7", "8", "9", "10", "17"]  # H Li C N O F Ne Cl
+ELEMENTS = ["1", "3", "6", "7", "8", "9", "10", "17", "29"]  # H Li C N O F Ne Cl Cu
 
 
 def main(src):

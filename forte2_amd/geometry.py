@@ -22,7 +22,8 @@ BOHR_TO_ANGSTROM = 0.529177210903
 ANGSTROM_TO_BOHR = 1.0 / BOHR_TO_ANGSTROM
 
 SYMBOLS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P",
-           "S", "Cl", "Ar"]
+           "S", "Cl", "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn",
+           "Ga", "Ge", "As", "Se", "Br", "Kr"]
 SYMBOL_TO_Z = {s.upper(): z for z, s in enumerate(SYMBOLS)}
 
 

@@ -9,6 +9,9 @@ reference's System(cholesky_tei=True), jkbuilder.py:90-117).
   the oracle's (dpstrf) B to <= 1e-10 Eh.
 * J/K of the device builder on that B equal the oracle's on the same B to
   <= 1e-10, and the precision form ends within 1e-8 Eh of FP64.
+* The RHF energies of N2 (tests/mcopt/test_casscf_cholesky_n2.py; along x and
+  along z, tests/scf/test_hf_sym.py) and cyclobutadiene
+  (tests/scf/test_hf_sym.py) at approx(abs=5e-8).
 """
 
 import json
@@ -23,6 +26,8 @@ from forte2_amd.system import System
 from oracle import cholesky as ochol
 from oracle import jk as ojk
 from oracle import scf as oscf
+
+from golden_systems import CHOLESKY_IDS, CHOLESKY_RHF
 
 pytestmark = pytest.mark.gpu
 
@@ -85,5 +90,15 @@ def test_device_rhf_with_cholesky_B_n2_golden(dev):
     s = System("N 0 0 0\nN 0 0 1.120", "cc-pvdz", cholesky_tei=True, cholesky_tol=g["cholesky_tol"], device=dev)
     r = RHF(charge=0, e_tol=1e-12)(s).run()
     print(f"[cholesky-gpu] N2: naux {s.naux} device {r.E:.12f} golden {g['E']:.12f}", flush=True)
+    assert r.converged and r.E == pytest.approx(g["E"], abs=5e-8)
+    s.fock_builder.release()
+
+
+@pytest.mark.parametrize("xyz,unit,key", CHOLESKY_RHF, ids=CHOLESKY_IDS)
+def test_device_rhf_with_cholesky_B_more_golden(dev, xyz, unit, key):
+    g = GOLD["energies"][key]
+    s = System(xyz, "cc-pvdz", cholesky_tei=True, cholesky_tol=g["cholesky_tol"], unit=unit, device=dev)
+    r = RHF(charge=0)(s).run()
+    print(f"[cholesky-gpu] {key}: nbf {s.nbf} naux {s.naux} device {r.E:.12f} golden {g['E']:.12f}", flush=True)
     assert r.converged and r.E == pytest.approx(g["E"], abs=5e-8)
     s.fock_builder.release()

@@ -174,14 +174,18 @@ def test_three_center_pair_blocks_match_square():
         ints.coulomb_3c_pairs_t(aux, b, 0, npairs, np.empty((int(col0[-1]) - 1, aux.size)))
 
 
-def test_three_center_pairs_by_aux_shells():
+@pytest.mark.parametrize("geom,orb,fit", [
+    ("O\nH 1 1.1\nH 1 1.1 2 104.5\nO 1 3.0 2 100.0 3 180.0", "cc-pvdz", "cc-pvtz-jkfit"),
+    ("Cu 0 0 0\nH 0 0 1.5", "cc-pvdz", "def2-universal-jkfit"),   # f orbital shells, aux shells up to l = 6
+], ids=["water_dimer", "CuH"])
+def test_three_center_pairs_by_aux_shells(geom, orb, fit):
     # f2ints_coulomb_3c_pairs_aux: the rows of the aux shells [s0, s1) of the
     # packed unique half, bit for bit; the scatter to (m, n) and (n, m)
     # rebuilds the square block (zero off the significant pairs in both)
     from forte2_amd.df import pair_columns
 
-    atoms = geo.parse_geometry("O\nH 1 1.1\nH 1 1.1 2 104.5\nO 1 3.0 2 100.0 3 180.0")
-    aux, b = bs.build_basis("cc-pvtz-jkfit", atoms), bs.build_basis("cc-pvdz", atoms)
+    atoms = geo.parse_geometry(geom)
+    aux, b = bs.build_basis(fit, atoms), bs.build_basis(orb, atoms)
     col0, m, n = pair_columns(b)
     npairs, nc = len(col0) - 1, int(col0[-1])
     whole = np.empty((aux.size, nc))

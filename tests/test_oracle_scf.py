@@ -26,15 +26,15 @@ H            0.000000000000     0.711620616369     0.489330954643"""
 NAPHTHALENE = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "naphthalene.xyz")).read()
 
 
-def _run(xyz, basis, aux, unit="angstrom", ms=None, guess="sap", **kw):
+def _run(xyz, basis, aux, unit="angstrom", ms=None, guess="sap", charge=0, **kw):
     s = System(xyz, basis, aux, unit=unit)
     B = ojk.build_B_density_fitting(s.coulomb_2c(), s.coulomb_3c())
     V_sap = s.sap_potential() if guess == "sap" else None
     jk = oscf.DFJK(B)
     if ms is None:
-        return oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, jk, s.Zsum,
+        return oscf.run_rhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, jk, s.Zsum - charge,
                             X=s.Xorth, V_sap=V_sap, **kw)
-    return oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, jk, s.Zsum, ms,
+    return oscf.run_uhf(s.ints_overlap(), s.ints_hcore(), s.nuclear_repulsion, jk, s.Zsum - charge, ms,
                         X=s.Xorth, V_sap=V_sap, **kw)
 
 
@@ -166,10 +166,16 @@ def test_uhf_triplet_golden():
 
 
 def test_uhf_singlet_equals_rhf():
-    # tests/scf/test_uhf.py:26-40 (ms = 0 UHF collapses to RHF)
+    # ms = 0 UHF collapses to RHF: here at cc-pVDZ against the RHF golden
+    # value, and the reference's own case (tests/scf/test_uhf.py:24-40) at
+    # cc-pVQZ
     r = _run(WATER, "cc-pvdz", "cc-pvtz-jkfit", ms=0.0, e_tol=1e-10)
     assert r.E == pytest.approx(EREF["water_dz_rhf"]["E"], abs=5e-8)
     assert abs(r.S2) < 1e-8
+    g = EREF["water_qz_uhf_singlet"]
+    r = _run(WATER, "cc-pvqz", "cc-pvqz-jkfit", ms=0.0)
+    assert r.E == pytest.approx(g["E"], abs=5e-8)
+    assert r.S2 == pytest.approx(g["S2"], abs=5e-8)
 
 
 def test_scf_failure_paths():

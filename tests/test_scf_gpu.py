@@ -109,8 +109,10 @@ def test_rhf_naphthalene_tz_golden(dev):
 @pytest.mark.parametrize("xyz,basis,aux,unit,key,kw", MORE_RHF, ids=[c[4] for c in MORE_RHF])
 def test_more_rhf_golden(dev, xyz, basis, aux, unit, key, kw):
     # RHF energies asserted by the reference's DSRG and CASSCF tests
+    kw = dict(kw)
+    charge = kw.pop("charge", 0)
     s = System(xyz, basis, aux, unit=unit, device=dev)
-    scf = RHF(charge=0, **kw)(s).run()
+    scf = RHF(charge=charge, **kw)(s).run()
     print(f"[scf-gpu] {key}: device {scf.E:.12f} golden {EREF[key]['E']:.12f}", flush=True)
     assert scf.converged
     assert scf.E == pytest.approx(EREF[key]["E"], abs=5e-8)
@@ -189,6 +191,16 @@ def test_uhf_guess_mix_golden(dev):
     u = UHF(charge=0, ms=0.0, guess_mix=True)(s).run()
     assert u.E == pytest.approx(EREF["h2_stretched_tz_uhf_guess_mix"]["E"], abs=5e-8)
     assert u.S2 == pytest.approx(EREF["h2_stretched_tz_uhf_guess_mix"]["S2"], abs=5e-8)
+
+
+def test_uhf_water_singlet_golden(dev):
+    # tests/scf/test_uhf.py:24-40: ms = 0 UHF of the cc-pVQZ water is its RHF
+    g = EREF["water_qz_uhf_singlet"]
+    s = System(WATER, "cc-pvqz", "cc-pvqz-jkfit", device=dev)
+    scf = UHF(charge=0, ms=0.0)(s).run()
+    print(f"   E = {scf.E:.12f} <S^2> = {scf.S2:.2e}", flush=True)
+    assert scf.E == pytest.approx(g["E"], abs=5e-8)
+    assert scf.S2 == pytest.approx(g["S2"], abs=5e-8)
 
 
 def test_uhf_water_triplet_golden(dev):
