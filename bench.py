@@ -221,32 +221,42 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
     out = {"workload": f"(H2O)_{n} RHF cc-pVDZ/cc-pVTZ-JKFIT, box {box} A, seed {seed}",
            "nbf": s.nbf, "naux": s.naux, "nocc": s.Zsum // 2, "B_formation_s": t_b,
            "B_formation_detail": fb.B_stats, "e_tol": 1e-10}
-    # one-time work outside the timed SCFs: the guess potential (cached on the
-    # System), the eigensolver's first call, one build of each precision (the
-    # context, the element scan of each B copy, the workspace)
+    # one-time work outside the timed SCFs: the guess potential and the core
+    # Hamiltonian (host integrals cached on the System), the eigensolver's
+    # first call, one build of each precision (the context, the element scan
+    # of each B copy, the workspace)
     s.sap_potential()
+    s.ints_hcore()
     torch.linalg.eigh(torch.eye(s.nbf, dtype=torch.float64, device=dev))
     Cw = torch.eye(s.nbf, s.Zsum // 2, dtype=torch.float64, device=dev)
     for prec in ("fp64", mixed):
         fb.build_JK([Cw], precision=prec)
     torch.cuda.synchronize()
     runs = {}
-    for prec in ("fp64", mixed):
+    # the plain FP64 SCF, the precision form (FP64 re-reference tail) and, for
+    # comparison, the precision form with FP64 cycles in the tail
+    for name, prec, tail in (("fp64", "fp64", "reref"), (mixed, mixed, "reref"),
+                             (mixed + "_fp64_cycle_tail", mixed, "full")):
         fb.timing = False
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        r = RHF(charge=0, e_tol=1e-10, precision=prec)(s)
+        lines = []
+        # the FP64-cycle-tail run also measures the low-precision build's
+        # energy error at the first reference density (one extra build)
+        r = RHF(charge=0, e_tol=1e-10, precision=prec, fp64_tail=tail, log=lines.append,
+                lp_error_check=(tail == "full"))(s)
         r.profile = profile
         r.run()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
         stages = "".join(h[1][0] for h in r.history)
-        runs[prec] = {"E": r.E, "iterations": r.iter, "scf_seconds": dt, "stages": stages,
+        runs[name] = {"E": r.E, "iterations": r.iter, "scf_seconds": dt, "stages": stages,
                       "converged": bool(r.converged), "lp_error_vs_fp64_Eh": r.lp_error_vs_fp64,
-                      "dE_per_iteration": [h[2] - g[2] for g, h in zip(r.history, r.history[1:])]}
+                      "dE_per_iteration": [h[2] - g[2] for g, h in zip(r.history, r.history[1:])],
+                      "iterations_log": lines}
         if profile:
-            runs[prec]["phase_seconds"] = r.phase_seconds
-        log(f"{prec}: E={r.E:.12f} iters={r.iter} {dt:.2f} s stages={stages}")
+            runs[name]["phase_seconds"] = r.phase_seconds
+        log(f"{name}: E={r.E:.12f} iters={r.iter} {dt:.2f} s stages={stages}")
     out["runs"] = runs
     # per-phase times of one build on this real B (the converged occupied
     # orbitals; J reads only the elements where B is nonzero for some P)
@@ -255,6 +265,9 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
     log("real-B builds (ms): " + json.dumps({p: round(v["total"], 2) for p, v in out["real_B_build_ms"].items()}))
     out["final_energy_error_vs_fp64_Eh"] = abs(runs[mixed]["E"] - runs["fp64"]["E"])
     out["scf_speedup_vs_fp64"] = runs["fp64"]["scf_seconds"] / runs[mixed]["scf_seconds"]
+    full = runs[mixed + "_fp64_cycle_tail"]
+    out["fp64_cycle_tail"] = {"final_energy_error_vs_fp64_Eh": abs(full["E"] - runs["fp64"]["E"]),
+                              "scf_speedup_vs_fp64": runs["fp64"]["scf_seconds"] / full["scf_seconds"]}
     fb.release()
     s.fock_builder = None
     del s, fb

@@ -15,16 +15,34 @@ early iterations run in that precision on full densities; when |dE| drops
 below ``switch_inc`` (3e-3), or the energy rises by less than ten times that
 (the low-precision energy has reached its noise floor: at (H2O)_64 a BF16x3
 build is 2e-3 Eh off FP64 and consecutive |dE| stall near 3e-4 with
-alternating signs), an FP64 reference Fock matrix F_ref(D_ref) is built once
+alternating signs), an FP64 reference Fock matrix F_ref(D_ref) is built
 and later iterations use incremental builds F = F_ref + G_lp(D - D_ref) whose
-error scales with |D - D_ref|; once |dE| < ``switch_fp64`` (1e-6) the
-remaining cycles, including the converged one, use FP64 builds.  The
-difference density of each spin is factored exactly as
+error scales with |D - D_ref|.  Once |dE| < ``switch_fp64`` (1e-6) the tail
+runs in one of two ways (``fp64_tail``):
+  * ``"reref"`` (default): one more FP64 reference build at the current
+    density, then incremental builds again until convergence.  |D - D_ref| is
+    now of the order of the remaining density change, so the low-precision
+    part of F is ~1e-5 of an already small G(D - D_ref).  That part is
+    rounded afresh every cycle (the factors of D - D_ref change), so the
+    incremental energies scatter by a few 1e-10 Eh at (H2O)_64 and the energy
+    test of these cycles is |dE| < max(e_tol, ``tail_e_tol``) (1e-9), with the
+    density test d_tol unchanged.  The converged iteration is followed, as in
+    every form, by the extra diagonalisation and an FP64 build, so the
+    reported energy is an FP64 energy at a density within ~|dF|/gap of the
+    FP64 one (its error is second order in that).  If these incremental
+    cycles have not converged after ``inc_max_tail`` iterations the remaining
+    ones use FP64 builds;
+  * ``"full"``: the remaining cycles, including the converged one, use FP64
+    builds.
+
+The difference density of each spin is factored exactly as
 D - D_ref = P A^T + A P^T with the reference orbitals rotated onto the new
 ones (orthogonal Procrustes), P their midpoint and A their difference, so
 K(D - D_ref) = (K[U+] - K[U-]) / 2 with U+- = sqrt(t) P +- A / sqrt(t).
-Every run records, per iteration, the precision used and, at the FP64
-reference build, the error of the low-precision energy against FP64.
+Every run records, per iteration, the build used ("lp", "ref", "inc",
+"fp64"); with ``lp_error_check`` it also builds the low-precision Fock
+matrix once more at the first reference density and records its energy
+error against FP64 (one extra low-precision build).
 """
 
 import time
@@ -109,13 +127,36 @@ class DeviceDIIS:
         return out
 
 
+def _polar(M, tol=5e-14, max_iter=24):
+    """Orthogonal polar factor of a square M (the R of the orthogonal
+    Procrustes problem).  M = Co^T S Cn has the cosines of the angles between
+    two occupied spaces as its singular values, near 1 in an SCF's late
+    cycles, where the Newton-Schulz iteration X <- X (3 I - X^T X) / 2
+    converges quadratically (a 1e-2 deviation from orthogonality reaches
+    roundoff in four steps) with small GEMMs only; checked every four steps.
+    When it has not reached ||X^T X - I||_max < tol the SVD gives the factor
+    (its latency on the device is ~40 ms at 320 x 320, the iteration's ~0.2 ms)."""
+    n = M.shape[0]
+    eye = torch.eye(n, dtype=M.dtype, device=M.device)
+    X = M
+    for k in range(max_iter):
+        X = 0.5 * X @ (3.0 * eye - X.T @ X)
+        if k % 4 == 3:
+            err = float((X.T @ X - eye).abs().max())
+            if not err < 1.0:  # diverging (a singular value beyond sqrt(3)) or NaN
+                break
+            if err < tol:
+                return X
+    U, _, Vh = torch.linalg.svd(M)
+    return U @ Vh
+
+
 def _procrustes_split(Cn, Co, S):
     """U+, U- with Cn Cn^T - Co Co^T = (U+ U+^T - U- U-^T) / 2."""
     if Cn.shape[1] == 0:
         return None
     M = Co.T @ S @ Cn
-    U, _, Vh = torch.linalg.svd(M)
-    Coa = Co @ (U @ Vh)
+    Coa = Co @ _polar(M)
     P = 0.5 * (Cn + Coa)
     A = Cn - Coa
     na, npn = torch.linalg.norm(A), torch.linalg.norm(P)
@@ -146,6 +187,7 @@ class _SCF:
     def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
                  d_tol=1e-6, maxiter=100, guess_type="minao", die_if_not_converged=True,
                  precision="fp64", switch_inc=3e-3, switch_fp64=1e-6, incremental=True,
+                 fp64_tail="reref", inc_max_tail=8, tail_e_tol=1e-9, lp_error_check=False,
                  device=None, log=None, level_shift=None, level_shift_thresh=1e-5):
         self.charge = charge
         self.level_shift, self.level_shift_thresh = level_shift, level_shift_thresh
@@ -157,6 +199,10 @@ class _SCF:
             raise ValueError(f"unknown precision {precision!r}")
         self.precision = precision
         self.switch_inc, self.switch_fp64, self.incremental = switch_inc, switch_fp64, incremental
+        if fp64_tail not in ("reref", "full"):
+            raise ValueError(f"unknown fp64_tail {fp64_tail!r}")
+        self.fp64_tail, self.inc_max_tail, self.tail_e_tol = fp64_tail, int(inc_max_tail), float(tail_e_tol)
+        self.lp_error_check = bool(lp_error_check)
         self.device = device
         self.log = log
         self.converged = False
@@ -271,7 +317,7 @@ class _SCF:
         Jd = (yield ("J", [sum(dD)] if len(dD) > 1 else dD, self.precision))[0]
         Kd = []
         for cn, co in zip(Cocc, Cref):
-            split = _procrustes_split(cn, co, self.S)
+            split = self._timed("split", _procrustes_split, cn, co, self.S)
             if split is None:
                 Kd.append(torch.zeros_like(Jd))
             else:
@@ -298,7 +344,7 @@ class _SCF:
         return out
 
     def run(self):
-        self.phase_seconds = {"build": 0.0, "diag": 0.0} if self.profile else None
+        self.phase_seconds = {"build": 0.0, "diag": 0.0, "split": 0.0} if self.profile else None
         gen = self.iterate()
         try:
             req = next(gen)
@@ -307,7 +353,7 @@ class _SCF:
         except StopIteration:
             pass
         if self.profile:
-            self.phase_seconds["other"] = self.time - self.phase_seconds["build"] - self.phase_seconds["diag"]
+            self.phase_seconds["other"] = self.time - sum(self.phase_seconds.values())
         return self
 
     def iterate(self):
@@ -329,28 +375,39 @@ class _SCF:
         self.history.append((self.iter, stage, self.E))
         self.lp_error_vs_fp64 = None
         self._current_level_shift = self.level_shift
+        tail, tail_inc = False, 0  # in the converging tail; incremental cycles in it
         for _ in range(self.maxiter):
             grad = self._ao_grad(F)
             Fx = self._apply_level_shift(self._diis_update(diis, F, grad))
             self.eps, self.C = self._timed("diag", self._diagonalize, Fx)
             self.D = self._density()
             # stage transitions of the precision form (decided on the previous dE)
-            next_stage = stage
+            next_stage = "inc" if stage == "ref" else stage
             if stage == "lp" and (abs(self._last_dE) < self.switch_inc
                                   or 0.0 < self._last_dE < 10.0 * self.switch_inc):
                 next_stage = "ref" if self.incremental else "fp64"
-            elif stage == "inc" and abs(self._last_dE) < self.switch_fp64:
-                next_stage = "fp64"
+            elif next_stage == "inc" and not tail and abs(self._last_dE) < self.switch_fp64:
+                tail = True
+                if self.fp64_tail == "full":
+                    next_stage = "fp64"
+                elif stage != "ref":  # a reference taken just now is the tail's reference
+                    next_stage = "ref"
+            elif next_stage == "inc" and tail:
+                tail_inc += 1
+                if tail_inc > self.inc_max_tail:
+                    next_stage = "fp64"
             if stage == "lp" and next_stage == "fp64":
                 diis.reset()  # low-precision Fock matrices leave the DIIS subspace
             F = yield from self._fock_gen(next_stage)
             self.E = self.Vnn + self._energy(F)
-            if next_stage == "ref":
-                # error of the low-precision build against FP64 at the same density
-                Flp = yield from self._fock_gen("lp")
-                self.lp_error_vs_fp64 = abs((self.Vnn + self._energy(Flp)) - self.E)
+            if next_stage == "ref" and not tail:
+                # first reference: the DIIS subspace restarts from
+                # FP64-accurate Fock matrices (a later re-reference keeps it)
+                if self.lp_error_check:
+                    # the low-precision build's energy error at this density
+                    Flp = yield from self._fock_gen("lp")
+                    self.lp_error_vs_fp64 = abs((self.Vnn + self._energy(Flp)) - self.E)
                 diis.reset()
-                next_stage = "inc"
             stage = next_stage
             dE = self.E - Eold
             self._last_dE = dE
@@ -360,7 +417,10 @@ class _SCF:
             self.history.append((self.iter + 1, stage, self.E))
             if self.log:
                 self.log(f"{self.iter + 1:4d} {self.E:20.12f} {dE:12.4e} {dD:12.4e} {stage:>5s} {diis.status}")
-            if stage == "fp64" and abs(dE) < self.e_tol and dD < self.d_tol:
+            # convergence is tested on FP64 builds and on the tail's cycles
+            # (incremental against the tail's reference, or that reference)
+            e_tol = max(self.e_tol, self.tail_e_tol) if stage != "fp64" else self.e_tol
+            if (stage == "fp64" or tail) and abs(dE) < e_tol and dD < self.d_tol:
                 # final iteration: diagonalise the new, non-extrapolated Fock
                 # matrix (scf_base.py:214-221; F_canon was rebuilt at :196)
                 self.eps, self.C = self._diagonalize(F)

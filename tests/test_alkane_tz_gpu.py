@@ -59,7 +59,7 @@ def test_uhf_alkane_tz_precision_form(dev, n):
     print(f"[alkane-gpu] C{n}H{2 * n + 2}+ bf16x3 {mix.E:.12f} fp64 {e64.E:.12f} diff {mix.E - e64.E:.2e} "
           f"stages {stages}", flush=True)
     assert mix.converged and abs(mix.E - e64.E) <= 1e-8
-    assert stages[0] == "l" and stages[-1] == "f"
+    assert stages[0] == "l" and "r" in stages and stages[-1] in "if"
     s.fock_builder.release()
 
 

@@ -118,13 +118,14 @@ def test_h2o64_rhf_precision_form_final_energy(dev):
     torch.cuda.synchronize()
     e64 = RHF(charge=0, e_tol=1e-10)(s).run()
     print(f"   fp64: E={e64.E:.12f} iters {e64.iter} {e64.time:.2f} s", flush=True)
-    mix = RHF(charge=0, e_tol=1e-10, precision="bf16x3")(s).run()
+    mix = RHF(charge=0, e_tol=1e-10, precision="bf16x3", lp_error_check=True)(s).run()
     stages = "".join(h[1][0] for h in mix.history)
     print(f"   bf16x3: E={mix.E:.12f} iters {mix.iter} {mix.time:.2f} s stages {stages} "
           f"|dE|={abs(mix.E - e64.E):.2e} lp-vs-fp64 {mix.lp_error_vs_fp64:.2e}", flush=True)
     assert e64.converged and mix.converged
     assert abs(mix.E - e64.E) <= 1e-8
-    assert stages[0] == "l" and "i" in stages and stages[-1] == "f"
+    # the final energy always comes from an FP64 build after the last cycle
+    assert stages[0] == "l" and "r" in stages and "i" in stages and stages[-1] in "if"
     fb.release()
     s.fock_builder = None
     del s, fb, e64, mix

@@ -22,21 +22,36 @@ H 0.000000000000 -0.711620616369 0.489330954643
 H 0.000000000000 0.711620616369 0.489330954643"""
 
 
+def _hilo(x):
+    """x rounded as the BF16x3 kernels see it: FP32, then split into a bf16
+    high part and a bf16 remainder (about 16 significant bits)."""
+    x32 = torch.as_tensor(x).to(torch.float32)
+    hi = x32.to(torch.bfloat16).to(torch.float32)
+    lo = (x32 - hi).to(torch.bfloat16).to(torch.float32)
+    return (hi + lo).to(torch.float64)
+
+
 class HostBuilder:
-    """build_JK / build_J / build_K over the oracle, on CPU torch tensors."""
+    """build_JK / build_J / build_K over the oracle, on CPU torch tensors.
+    "fp32" rounds B to FP32; "bf16x3" rounds B and the orbital factors to
+    hi + lo bf16 pairs, the operand rounding of the BF16x3 products."""
 
     def __init__(self, B):
         self.B = B
         self.B32 = B.astype(np.float32).astype(np.float64)
+        self.B16 = _hilo(B).numpy()
         self.device = torch.device("cpu")
         self.calls = []
 
     def _B(self, precision):
-        return self.B if precision == "fp64" else self.B32
+        return {"fp64": self.B, "fp32": self.B32, "bf16x3": self.B16}[precision]
+
+    def _C(self, C, precision):
+        return [(_hilo(c) if precision == "bf16x3" else c).numpy() for c in C]
 
     def build_JK(self, C, precision="fp64"):
         self.calls.append(("JK", precision))
-        J, K = ojk.build_JK(self._B(precision), [c.numpy() for c in C])
+        J, K = ojk.build_JK(self._B(precision), self._C(C, precision))
         return [torch.from_numpy(j) for j in J], [torch.from_numpy(k) for k in K]
 
     def build_J(self, D, precision="fp64"):
@@ -45,7 +60,7 @@ class HostBuilder:
 
     def build_K(self, C, precision="fp64"):
         self.calls.append(("K", precision))
-        return [torch.from_numpy(k) for k in ojk.build_K_nPm(self._B(precision), [c.numpy() for c in C])]
+        return [torch.from_numpy(k) for k in ojk.build_K_nPm(self._B(precision), self._C(C, precision))]
 
 
 @pytest.fixture(scope="module")
@@ -120,15 +135,62 @@ def test_precision_schedule_and_requests(water):
     s, B = water
     hb = HostBuilder(B)
     s.fock_builder = hb
-    r = RHF(charge=0, e_tol=1e-10, precision="fp32")(s).run()
+    r = RHF(charge=0, e_tol=1e-10, precision="fp32", lp_error_check=True)(s).run()
     e64 = RHF(charge=0, e_tol=1e-10)(s).run().E
     stages = "".join(h[1][0] for h in r.history)
-    assert stages.startswith("l") and "i" in stages and stages.endswith("f")
+    # low precision, the first FP64 reference, incremental cycles, one FP64
+    # re-reference, incremental cycles to convergence (then the FP64 final build)
+    assert stages.startswith("l") and stages.count("r") == 2 and stages.endswith("i")
+    assert stages.index("r") < stages.index("i") < stages.rindex("r")
     assert abs(r.E - e64) < 1e-9
     assert r.lp_error_vs_fp64 is not None and r.lp_error_vs_fp64 > 0
     kinds = {c for c in hb.calls}
     assert ("JK", "fp32") in kinds and ("J", "fp32") in kinds and ("K", "fp32") in kinds
     assert ("JK", "fp64") in kinds
+
+
+@pytest.mark.parametrize("tail", ["reref", "full"])
+def test_bf16x3_like_tails_end_at_the_fp64_energy(water, tail):
+    """Both tails of the precision form, on a builder with the BF16x3 operand
+    rounding: the final energy is within 1e-9 Eh of the FP64 SCF's, and the
+    re-reference tail makes fewer FP64 builds than the FP64-cycle tail."""
+    s, B = water
+    hb = HostBuilder(B)
+    s.fock_builder = hb
+    e64 = RHF(charge=0, e_tol=1e-10)(s).run().E
+    hb.calls.clear()
+    r = RHF(charge=0, e_tol=1e-10, precision="bf16x3", fp64_tail=tail, lp_error_check=True)(s).run()
+    stages = "".join(h[1][0] for h in r.history)
+    n64 = sum(1 for c in hb.calls if c[1] == "fp64")
+    print(f"tail {tail}: stages {stages} FP64 builds {n64} |dE| {abs(r.E - e64):.1e}")
+    assert r.converged and abs(r.E - e64) < 1e-9
+    assert r.lp_error_vs_fp64 > 1e-7  # the emulated rounding is BF16x3-sized
+    if tail == "reref":
+        assert stages.count("r") == 2 and "f" not in stages and n64 == 3
+    else:
+        assert stages.count("r") == 1 and stages.endswith("f") and n64 == 1 + stages.count("f") + 1
+    # without the check no extra low-precision build is made
+    hb.calls.clear()
+    r2 = RHF(charge=0, e_tol=1e-10, precision="bf16x3", fp64_tail=tail)(s).run()
+    assert r2.lp_error_vs_fp64 is None and r2.E == r.E
+    # the first build, one JK per lp/ref/fp64 cycle, J + K per incremental
+    # cycle, the final FP64 build
+    assert len(hb.calls) == 1 + sum(2 if h[1] == "inc" else 1 for h in r2.history[1:]) + 1
+
+
+def test_tail_falls_back_to_fp64_cycles(water):
+    """An incremental tail that has not converged within inc_max_tail cycles
+    hands over to FP64 cycles."""
+    s, B = water
+    s.fock_builder = HostBuilder(B)
+    e64 = RHF(charge=0, e_tol=1e-10)(s).run().E
+    r = RHF(charge=0, e_tol=1e-10, precision="bf16x3", inc_max_tail=0)(s).run()
+    stages = "".join(h[1][0] for h in r.history)
+    assert stages.count("r") == 2 and stages.endswith("f")
+    assert stages[stages.rindex("r") + 1] == "f"  # no incremental cycle after the re-reference
+    assert r.converged and abs(r.E - e64) < 1e-9
+    with pytest.raises(ValueError, match="fp64_tail"):
+        RHF(charge=0, fp64_tail="fp32")
 
 
 def test_generator_protocol_yields_build_requests(water):

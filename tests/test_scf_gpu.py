@@ -221,18 +221,22 @@ def test_uhf_alkane_cation_plain_form(dev):
     assert abs(scf.S2 - ref.S2) <= 1e-7
 
 
+@pytest.mark.parametrize("tail", ["reref", "full"])
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 @pytest.mark.parametrize("n", [1, 16])
-def test_precision_form_final_energy(dev, precision, n):
+def test_precision_form_final_energy(dev, precision, n, tail):
     xyz = WATER if n == 1 else geo.water_cluster(16, 9.0, seed=16)
     s = System(xyz, "cc-pvdz", "cc-pvtz-jkfit", device=dev)
     e64 = RHF(charge=0, e_tol=1e-10)(s).run()
-    mix = RHF(charge=0, e_tol=1e-10, precision=precision)(s).run()
+    mix = RHF(charge=0, e_tol=1e-10, precision=precision, fp64_tail=tail, lp_error_check=True)(s).run()
     stages = [h[1] for h in mix.history]
-    print(f"[scf-gpu] (H2O)_{n} {precision}: E {mix.E:.12f} fp64 {e64.E:.12f} diff {mix.E - e64.E:.2e} "
-          f"stages {''.join(x[0] for x in stages)} lp-vs-fp64 {mix.lp_error_vs_fp64}", flush=True)
-    assert abs(mix.E - e64.E) <= 1e-8
-    assert stages[0] == "lp" and stages[-1] == "fp64"
+    print(f"[scf-gpu] (H2O)_{n} {precision} tail {tail}: E {mix.E:.12f} fp64 {e64.E:.12f} "
+          f"diff {mix.E - e64.E:.2e} stages {''.join(x[0] for x in stages)} "
+          f"lp-vs-fp64 {mix.lp_error_vs_fp64}", flush=True)
+    assert mix.converged and abs(mix.E - e64.E) <= 1e-8
+    assert stages[0] == "lp" and "ref" in stages
+    # the energy reported is that of the FP64 build after the last cycle
+    assert stages[-1] in (("fp64",) if tail == "full" else ("inc", "ref", "fp64"))
     assert mix.lp_error_vs_fp64 is not None
 
 

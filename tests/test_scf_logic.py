@@ -109,3 +109,32 @@ def test_diis_coefficients_minimise_the_error_on_the_affine_hull(impl):
         assert abs(w.sum() - 1.0) < 1e-12
         ref = _affine_min(np.array(errs[lo:k + 1]).T)
         assert np.abs(w - ref).max() < 1e-8 * max(1.0, np.abs(ref).max())
+
+
+def test_polar_factor_matches_the_svd_and_falls_back():
+    """The Newton-Schulz polar factor of the difference-density split equals
+    U Vh of the SVD (the orthogonal Procrustes solution) for a near-orthogonal
+    M, is orthogonal to roundoff, and a singular M takes the SVD path."""
+    from forte2_amd.scf import _polar, _procrustes_split
+
+    g = torch.Generator().manual_seed(5)
+    n = 40
+    Q, _ = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, generator=g))
+    for eps in (1e-6, 1e-3, 5e-2):
+        M = Q + eps * torch.randn(n, n, dtype=torch.float64, generator=g)
+        U, _, Vh = torch.linalg.svd(M)
+        R = _polar(M)
+        assert torch.allclose(R, U @ Vh, atol=1e-12, rtol=0)
+        assert float((R.T @ R - torch.eye(n, dtype=torch.float64)).abs().max()) < 5e-14
+    Msing = Q.clone()
+    Msing[:, 0] = 0.0  # a 90-degree rotation: singular, Newton-Schulz cannot converge
+    R = _polar(Msing)
+    assert float((R.T @ R - torch.eye(n, dtype=torch.float64)).abs().max()) < 1e-12
+    # the split's identity with the polar factor: Cn Cn^T - Co Co^T = (U+U+^T - U-U-^T) / 2
+    N, o = 60, 12
+    S = torch.eye(N, dtype=torch.float64)
+    Co, _ = torch.linalg.qr(torch.randn(N, o, dtype=torch.float64, generator=g))
+    Cn, _ = torch.linalg.qr(Co + 1e-2 * torch.randn(N, o, dtype=torch.float64, generator=g))
+    Up, Um = _procrustes_split(Cn, Co, S)
+    lhs = Cn @ Cn.T - Co @ Co.T
+    assert torch.allclose(lhs, 0.5 * (Up @ Up.T - Um @ Um.T), atol=1e-14, rtol=0)
