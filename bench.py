@@ -231,6 +231,11 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
     Cw = torch.eye(s.nbf, s.Zsum // 2, dtype=torch.float64, device=dev)
     for prec in ("fp64", mixed):
         fb.build_JK([Cw], precision=prec)
+    # and the first call of every device routine the SCF loop uses (DIIS,
+    # gradient, energy, the difference-density split): two short SCFs whose
+    # results are discarded
+    for prec, it in (("fp64", 2), (mixed, 12)):
+        RHF(charge=0, precision=prec, maxiter=it, die_if_not_converged=False)(s).run()
     torch.cuda.synchronize()
     runs = {}
     # the plain FP64 SCF, the precision form (FP64 re-reference tail) and, for
