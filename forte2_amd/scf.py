@@ -30,8 +30,13 @@ runs in one of two ways (``fp64_tail``):
     every form, by the extra diagonalisation and an FP64 build, so the
     reported energy is an FP64 energy at a density within ~|dF|/gap of the
     FP64 one (its error is second order in that).  If these incremental
-    cycles have not converged after ``inc_max_tail`` iterations the remaining
-    ones use FP64 builds;
+    cycles have not converged ``inc_max_tail`` (8) cycles after the tail's
+    reference, one more FP64 reference is taken at the current density, up
+    to ``tail_refs`` (1) references in the tail; after that the remaining
+    cycles use FP64 builds.  (A UHF cation with a small gap, C4H10+ in
+    cc-pVTZ, keeps its density change near 1e-5 under the rounding of the
+    incremental cycles whatever the reference, so it needs FP64 cycles at
+    the end; a second reference only adds a build there.);
   * ``"full"``: the remaining cycles, including the converged one, use FP64
     builds.
 
@@ -187,7 +192,7 @@ class _SCF:
     def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
                  d_tol=1e-6, maxiter=100, guess_type="minao", die_if_not_converged=True,
                  precision="fp64", switch_inc=3e-3, switch_fp64=1e-6, incremental=True,
-                 fp64_tail="reref", inc_max_tail=8, tail_e_tol=1e-9, lp_error_check=False,
+                 fp64_tail="reref", inc_max_tail=8, tail_refs=1, tail_e_tol=1e-9, lp_error_check=False,
                  device=None, log=None, level_shift=None, level_shift_thresh=1e-5):
         self.charge = charge
         self.level_shift, self.level_shift_thresh = level_shift, level_shift_thresh
@@ -202,6 +207,7 @@ class _SCF:
         if fp64_tail not in ("reref", "full"):
             raise ValueError(f"unknown fp64_tail {fp64_tail!r}")
         self.fp64_tail, self.inc_max_tail, self.tail_e_tol = fp64_tail, int(inc_max_tail), float(tail_e_tol)
+        self.tail_refs = int(tail_refs)
         self.lp_error_check = bool(lp_error_check)
         self.device = device
         self.log = log
@@ -375,7 +381,9 @@ class _SCF:
         self.history.append((self.iter, stage, self.E))
         self.lp_error_vs_fp64 = None
         self._current_level_shift = self.level_shift
-        tail, tail_inc = False, 0  # in the converging tail; incremental cycles in it
+        # in the converging tail; its incremental cycles since its last
+        # reference; the references taken in it
+        tail, tail_inc, tail_ref_count = False, 0, 0
         for _ in range(self.maxiter):
             grad = self._ao_grad(F)
             Fx = self._apply_level_shift(self._diis_update(diis, F, grad))
@@ -390,12 +398,17 @@ class _SCF:
                 tail = True
                 if self.fp64_tail == "full":
                     next_stage = "fp64"
-                elif stage != "ref":  # a reference taken just now is the tail's reference
+                elif stage != "ref":
                     next_stage = "ref"
+                else:  # a reference taken just now is the tail's reference
+                    tail_ref_count = 1
             elif next_stage == "inc" and tail:
                 tail_inc += 1
                 if tail_inc > self.inc_max_tail:
-                    next_stage = "fp64"
+                    next_stage = "ref" if tail_ref_count < self.tail_refs else "fp64"
+            if next_stage == "ref" and tail:
+                tail_ref_count += 1
+                tail_inc = 0
             if stage == "lp" and next_stage == "fp64":
                 diis.reset()  # low-precision Fock matrices leave the DIIS subspace
             F = yield from self._fock_gen(next_stage)

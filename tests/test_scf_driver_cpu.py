@@ -4,6 +4,7 @@ protocol without a GPU.  The builder's "low precision" rounds B to FP32."""
 
 import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -176,6 +177,20 @@ def test_bf16x3_like_tails_end_at_the_fp64_energy(water, tail):
     # the first build, one JK per lp/ref/fp64 cycle, J + K per incremental
     # cycle, the final FP64 build
     assert len(hb.calls) == 1 + sum(2 if h[1] == "inc" else 1 for h in r2.history[1:]) + 1
+
+
+def test_tail_takes_another_reference_before_fp64_cycles(water):
+    """With tail_refs=2 a stalled tail takes one more FP64 reference, then
+    FP64 cycles."""
+    s, B = water
+    s.fock_builder = HostBuilder(B)
+    e64 = RHF(charge=0, e_tol=1e-10)(s).run().E
+    r = RHF(charge=0, e_tol=1e-10, precision="bf16x3", inc_max_tail=1, tail_refs=2)(s).run()
+    stages = "".join(h[1][0] for h in r.history)
+    # low precision, the first reference, incremental cycles to |dE| < 1e-6,
+    # then the tail: reference, one cycle, reference, one cycle, FP64 cycles
+    assert re.fullmatch(r"l+ri+ririf+", stages), stages
+    assert r.converged and abs(r.E - e64) < 1e-9
 
 
 def test_tail_falls_back_to_fp64_cycles(water):
