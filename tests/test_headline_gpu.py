@@ -11,8 +11,8 @@
 * Size-independent properties at full size: J and K exactly symmetric;
   J linear in D; tr(D K) = sum_P ||C^T B_P C||_F^2 >= 0.
 * The real (H2O)_64 RHF (seed 64, 14.5 A cube): the precision form (BF16x3
-  builds, incremental Fock builds, FP64 final cycles) ends within 1e-8 Eh of
-  the FP64 SCF on the same integrals.
+  builds, incremental Fock builds against FP64 references, an FP64 final
+  build) ends within 1e-8 Eh of the FP64 SCF on the same integrals.
 
 BF16x3 K bound.  Each FP32 operand is carried as hi + lo with hi, lo bf16
 (8 significant bits each, unit roundoff u = 2^-8): |x - hi - lo| <= u^2 |x|,
