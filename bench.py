@@ -106,6 +106,8 @@ def parse():
                     help="instead of the J/K microbenchmark, run a real (H2O)_n RHF in FP64 and in the "
                          "precision form and report energies, errors and SCF latency")
     ap.add_argument("--seed", type=int, default=None, help="cluster seed for --scf (default n)")
+    ap.add_argument("--scf-alkane", type=int, default=None,
+                    help="--scf: the C_nH_(2n+2)+ UHF in cc-pVTZ/cc-pVTZ-JKFIT instead of (H2O)_n")
     ap.add_argument("--scf-profile", action="store_true",
                     help="--scf: also time the SCF phases (builds, diagonalisation, rest; device syncs added)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -186,16 +188,19 @@ def measured_peaks(read_buf=None, read_reps=3):
     return out
 
 
-def scf_leg(n, mixed, seed=None, log=None, profile=False):
-    """Real (H2O)_n cc-pVDZ/cc-pVTZ-JKFIT RHF: integrals on the host streamed into
-    HBM, then the FP64 plain form and the precision form on the same B, both
-    to e_tol 1e-10.  Returns energies, iterations, SCF seconds and the final
-    energy error of the precision form against FP64."""
+def scf_leg(n, mixed, seed=None, log=None, profile=False, alkane=None):
+    """Real (H2O)_n cc-pVDZ/cc-pVTZ-JKFIT RHF (or, with `alkane`, the
+    C_nH_(2n+2)+ cation UHF in cc-pVTZ/cc-pVTZ-JKFIT): integrals on the host
+    streamed into HBM, then the FP64 plain form and the precision form on
+    the same B, both to e_tol 1e-10.  Returns energies, iterations, SCF
+    seconds and the final energy error of the precision form against FP64."""
+    import functools
+
     import torch
 
     from forte2_amd import geometry as geo
     from forte2_amd import integrals
-    from forte2_amd.scf import RHF
+    from forte2_amd.scf import RHF, UHF
     from forte2_amd.system import System
 
     log = log or (lambda msg: None)
@@ -204,7 +209,14 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
     seed = n if seed is None else seed
     integrals.set_threads(host_cores())
     t0 = time.perf_counter()
-    s = System(geo.water_cluster(n, box, seed=seed), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+    if alkane:
+        s = System(geo.alkane(alkane), "cc-pvtz", "cc-pvtz-jkfit", device=dev)
+        SCF = functools.partial(UHF, 1, ms=0.5)
+        name = f"C{alkane}H{2 * alkane + 2}+ UHF (ms 0.5) cc-pVTZ/cc-pVTZ-JKFIT"
+    else:
+        s = System(geo.water_cluster(n, box, seed=seed), "cc-pvdz", "cc-pvtz-jkfit", device=dev)
+        SCF = functools.partial(RHF, 0)
+        name = f"(H2O)_{n} RHF cc-pVDZ/cc-pVTZ-JKFIT, box {box} A, seed {seed}"
     fb = s.fock_builder
     last = [0.0]
 
@@ -217,8 +229,8 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
     fb.B_Pmn
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
-    log(f"(H2O)_{n} N={s.nbf} Q={s.naux}: B formed in {t_b:.1f} s")
-    out = {"workload": f"(H2O)_{n} RHF cc-pVDZ/cc-pVTZ-JKFIT, box {box} A, seed {seed}",
+    log(f"{name}: N={s.nbf} Q={s.naux}: B formed in {t_b:.1f} s")
+    out = {"workload": name,
            "nbf": s.nbf, "naux": s.naux, "nocc": s.Zsum // 2, "B_formation_s": t_b,
            "B_formation_detail": fb.B_stats, "e_tol": 1e-10}
     # one-time work outside the timed SCFs: the guess potential and the core
@@ -235,7 +247,7 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
     # gradient, energy, the difference-density split): two short SCFs whose
     # results are discarded
     for prec, it in (("fp64", 2), (mixed, 12)):
-        RHF(charge=0, precision=prec, maxiter=it, die_if_not_converged=False)(s).run()
+        SCF(precision=prec, maxiter=it, die_if_not_converged=False)(s).run()
     torch.cuda.synchronize()
     runs = {}
     # the plain FP64 SCF, the precision form (FP64 re-reference tail) and, for
@@ -248,7 +260,7 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
         lines = []
         # the FP64-cycle-tail run also measures the low-precision build's
         # energy error at the first reference density (one extra build)
-        r = RHF(charge=0, e_tol=1e-10, precision=prec, fp64_tail=tail, log=lines.append,
+        r = SCF(e_tol=1e-10, precision=prec, fp64_tail=tail, log=lines.append,
                 lp_error_check=(tail == "full"))(s)
         r.profile = profile
         r.run()
@@ -282,7 +294,8 @@ def scf_leg(n, mixed, seed=None, log=None, profile=False):
 
 def scf_mode(args):
     out = scf_leg(args.waters, args.mixed, args.seed,
-                  log=lambda m: print(f"[scf] {m}", file=sys.stderr, flush=True), profile=args.scf_profile)
+                  log=lambda m: print(f"[scf] {m}", file=sys.stderr, flush=True), profile=args.scf_profile,
+                  alkane=args.scf_alkane)
     print(json.dumps(out))
 
 

@@ -24,13 +24,14 @@ runs in one of two ways (``fp64_tail``):
     now of the order of the remaining density change, so the low-precision
     part of F is ~1e-5 of an already small G(D - D_ref).  That part is
     rounded afresh every cycle (the factors of D - D_ref change), so the
-    incremental energies scatter by a few 1e-10 Eh at (H2O)_64 and the energy
-    test of these cycles is |dE| < max(e_tol, ``tail_e_tol``) (1e-9), with the
-    density test d_tol unchanged.  The converged iteration is followed, as in
-    every form, by the extra diagonalisation and an FP64 build, so the
-    reported energy is an FP64 energy at a density within ~|dF|/gap of the
-    FP64 one (its error is second order in that).  If these incremental
-    cycles have not converged ``inc_max_tail`` (8) cycles after the tail's
+    incremental energies scatter (a few 1e-10 Eh at (H2O)_64, up to ~1e-8 Eh
+    for alkane cations in cc-pVTZ) and the energy test of these cycles is
+    |dE| < max(e_tol, ``tail_e_tol``) (1e-8, the accuracy asked of the final
+    energy), with the density test d_tol unchanged.  The converged
+    iteration is followed, as in every form, by the extra diagonalisation
+    and an FP64 build, so the reported energy is an FP64 energy at a density
+    within ~|dF|/gap of the FP64 one (its error is second order in that).  If these incremental
+    cycles have not converged ``inc_max_tail`` (12) cycles after the tail's
     reference, one more FP64 reference is taken at the current density, up
     to ``tail_refs`` (1) references in the tail; after that the remaining
     cycles use FP64 builds.  (A UHF cation with a small gap, C4H10+ in
@@ -192,7 +193,7 @@ class _SCF:
     def __init__(self, charge, do_diis=True, diis_start=1, diis_nvec=8, diis_min=2, e_tol=1e-9,
                  d_tol=1e-6, maxiter=100, guess_type="minao", die_if_not_converged=True,
                  precision="fp64", switch_inc=3e-3, switch_fp64=1e-6, incremental=True,
-                 fp64_tail="reref", inc_max_tail=8, tail_refs=1, tail_e_tol=1e-9, lp_error_check=False,
+                 fp64_tail="reref", inc_max_tail=12, tail_refs=1, tail_e_tol=1e-8, lp_error_check=False,
                  device=None, log=None, level_shift=None, level_shift_thresh=1e-5):
         self.charge = charge
         self.level_shift, self.level_shift_thresh = level_shift, level_shift_thresh
