@@ -18,8 +18,11 @@ aux-slice sample, scaled).  Two more legs of the metric run outside the
 timed region:
   * accuracy, before the synthetic B is made (so that its B formation runs on
     device memory nothing has used yet): the real (H2O)_64 RHF (seed 64,
-    14.5 A cube) in FP64 and in the precision form;
-    ``final_energy_error_vs_fp64_Eh`` (bar: 1e-8 Eh);
+    14.5 A cube) in FP64, in the precision form, and in the precision form
+    with FP64 cycles in its tail for comparison;
+    ``final_energy_error_vs_fp64_Eh`` (bar: 1e-8 Eh) and the SCF speed-up
+    (``--scf`` runs this leg alone; ``--scf-alkane N`` the C_nH_(2n+2)+ UHF
+    in cc-pVTZ instead);
   * latency, after the synthetic B has left HBM: the first 48 requests of the
     seeded (H2O)_n serving stream
     (n ~ U{1..24}) at 2 requests/s through the continuous-batching server;
