@@ -45,26 +45,35 @@ def _free():
     torch.cuda.empty_cache()
 
 
-def _oracle_by_slices(B, Ch, pblk=256):
-    """Oracle J and K of the whole B, summed over P-slices copied to the host."""
+def _oracle_by_slices(B, Ch, say, pblk=256):
+    """Oracle J and K of the whole B, summed over P-slices copied to the host
+    (build_J and the (P,m,n)-layout K, which the oracle tests pin equal to
+    the transposed-layout K)."""
     Q = B.shape[0]
+    Dh = Ch @ Ch.T
     Jr = Kr = None
     t0 = time.perf_counter()
     for p0 in range(0, Q, pblk):
         p1 = min(Q, p0 + pblk)
         Bs = B[p0:p1].cpu().numpy()
-        J, K = ojk.build_JK(Bs, [Ch])
-        Jr = J[0] if Jr is None else Jr + J[0]
-        Kr = K[0] if Kr is None else Kr + K[0]
+        J, K = ojk.build_J(Bs, [Dh])[0], ojk.build_K_Pmn(Bs, [Ch])[0]
+        Jr = J if Jr is None else Jr + J
+        Kr = K if Kr is None else Kr + K
         del Bs
-        if (p0 // pblk) % 8 == 0:
-            print(f"   oracle slices: P < {p1} of {Q} ({time.perf_counter() - t0:.0f} s)", flush=True)
+        if (p0 // pblk) % 4 == 0:
+            say(f"   oracle slices: P < {p1} of {Q} ({time.perf_counter() - t0:.0f} s)")
     return Jr, Kr
 
 
-def test_h2o64_builds_match_the_oracle_at_full_size(dev):
+def test_h2o64_builds_match_the_oracle_at_full_size(dev, capsys):
+    # progress lines go to the terminal even under output capture: the
+    # oracle part of this test runs for a minute or more
+    def say(msg):
+        with capsys.disabled():
+            print(msg, flush=True)
+
     N, Q, o = water_cluster_shape(64)
-    print(f"[headline] (H2O)_64 shapes N={N} Q={Q} o={o}: synthetic B in HBM", flush=True)
+    say(f"[headline] (H2O)_64 shapes N={N} Q={Q} o={o}: synthetic B in HBM")
     B = synthetic_B(Q, N, seed=64, device=dev)
     C = orthonormal_C(N, o, seed=64, device=dev)
     fb = FockBuilder(B=B, device=dev)
@@ -72,7 +81,7 @@ def test_h2o64_builds_match_the_oracle_at_full_size(dev):
     Jm, Km = fb.build_JK([C], precision="bf16x3")
     torch.cuda.synchronize()
     J64, K64, Jm, Km = J64[0], K64[0], Jm[0], Km[0]
-    print("[headline] device builds done; properties", flush=True)
+    say("[headline] device builds done; properties")
 
     # exact symmetry (both triangles are written from one value)
     for X in (J64, K64, Jm, Km):
@@ -85,19 +94,19 @@ def test_h2o64_builds_match_the_oracle_at_full_size(dev):
     D1, D2 = C1 @ C1.T, C2 @ C2.T
     Ja, Jb, Jab = fb.build_J([D1, D2, D1 + D2])
     lin = float((Jab - Ja - Jb).abs().max())
-    print(f"   |J(D1+D2) - J(D1) - J(D2)| = {lin:.2e}", flush=True)
+    say(f"   |J(D1+D2) - J(D1) - J(D2)| = {lin:.2e}")
     assert lin <= TOL_FP64
     # J(D) from the density equals J from the occupied orbitals
     assert float((Jab - J64).abs().max()) <= TOL_FP64
     del Ja, Jb, Jab, D, D1, D2
 
-    Jr, Kr = _oracle_by_slices(B, C.cpu().numpy())
+    Jr, Kr = _oracle_by_slices(B, C.cpu().numpy(), say)
     ej = float(np.abs(J64.cpu().numpy() - Jr).max())
     ek = float(np.abs(K64.cpu().numpy() - Kr).max())
     rj = float(np.abs(Jm.cpu().numpy() - Jr).max() / np.abs(Jr).max())
     rk = float(np.abs(Km.cpu().numpy() - Kr).max() / np.abs(Kr).max())
-    print(f"   fp64: |dJ|={ej:.2e} |dK|={ek:.2e} (max|J|={np.abs(Jr).max():.2e}, max|K|={np.abs(Kr).max():.2e}); "
-          f"bf16x3: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}", flush=True)
+    say(f"   fp64: |dJ|={ej:.2e} |dK|={ek:.2e} (max|J|={np.abs(Jr).max():.2e}, max|K|={np.abs(Kr).max():.2e}); "
+        f"bf16x3: rel|dJ|={rj:.2e} rel|dK|={rk:.2e}")
     assert ej <= TOL_FP64 and ek <= TOL_FP64
     assert rj <= REL_J_BF16X3 and rk <= REL_K_BF16X3
     fb.release()
