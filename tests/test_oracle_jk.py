@@ -129,3 +129,22 @@ def test_mo_blocks_physicist_convention():
     V = ojk.two_electron_integrals_gen_block(B, eye, eye, eye, eye)
     assert np.abs(V - ojk.eri_from_B(B).transpose(0, 2, 1, 3)).max() < 1e-13
     assert np.abs(ojk.B_tensor_gen_block(B, eye, eye) - B).max() == 0.0
+
+
+def test_blas_forms_equal_the_reference_einsum_strings():
+    """build_J and build_K_Pmn are written as tensordot/GEMM contractions; on a
+    non-symmetric B and D (so that a swapped index shows) they equal the
+    reference's einsum strings (jkbuilder.py:161-166, 174-181)."""
+    rng = np.random.default_rng(21)
+    Q, N, o = 7, 9, 4
+    B = rng.standard_normal((Q, N, N))
+    D = rng.standard_normal((N, N))
+    C = rng.standard_normal((N, o))
+    J = ojk.build_J(B, [D])[0]
+    assert np.abs(J - np.einsum("Pmn,Prs,sr->mn", B, B, D)).max() < 1e-12
+    # the transposed D gives a different J on this B: the sr order matters
+    assert np.abs(J - ojk.build_J(B, [D.T])[0]).max() > 1e-3
+    Y = np.einsum("Pms,si->Pmi", B, C)
+    K = ojk.build_K_Pmn(B, [C])[0]
+    assert np.abs(K - np.einsum("Pmi,Pni->mn", Y, Y)).max() < 1e-12
+    assert ojk.build_K_Pmn(B, [C[:, :0]])[0].shape == (N, N)
